@@ -1,0 +1,202 @@
+"""Benchmark: DG right-hand side throughput on one MI355X.
+
+Workload (one "step" = one evaluation of the full right-hand side):
+uniform 64 x 64 x 8 octree, order N = 4 (125 LGL nodes per element,
+4,096,000 DOF per field), 100 km square, tanh shelf 20 -> 200 m, seeded
+Fourier state; FP64 storage and arithmetic (the parity path), device-resident
+structure-of-arrays state, Coriolis f0 = 1e-4, beta = 0.
+
+value = GDOF/s = E * (N+1)^3 * 5 / t_RHS / 1e9 with inputs already in HBM.
+Extras: the mixed-precision (FP32 volume) kernel with its relative L2 error
+against the FP64 result, fused / un-fused RK4 steps, and the oracle's FP64
+CPU baseline timed on a bounded sample of elements.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_launches(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        fn()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps  # ms per call
+
+
+def rel_l2_fields(a, b, E, nd):
+    import numpy as np
+
+    A = a.reshape(5, E, nd)
+    B = b.reshape(5, E, nd)
+    return [float(np.linalg.norm(A[f] - B[f]) / np.linalg.norm(B[f])) for f in range(5)]
+
+
+def cpu_baseline(case, target_s=12.0):
+    """Oracle (dense Kronecker mat-vecs, the reference algorithm) on 1 thread
+    over a bounded prefix of the mesh's elements."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    f = np.full(case.h.size, 1e-4)
+    nd = case.nd
+    probe = 256
+    t = time.perf_counter()
+    oracle.rhs(case.order, case.U, case.h, f, e0=0, e1=probe)
+    per = (time.perf_counter() - t) / probe
+    ns = int(min(case.E, max(probe, target_s / max(per, 1e-9))))
+    t = time.perf_counter()
+    oracle.rhs(case.order, case.U, case.h, f, e0=0, e1=ns)
+    dt = time.perf_counter() - t
+    return {
+        "value": ns * nd * 5 / dt / 1e9,
+        "unit": "GDOF/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{ns} of {case.E} elements of the 64x64x8 N=4 mesh, one RHS "
+                  f"({dt:.1f} s), dense Kronecker mat-vecs as the reference computes "
+                  "them, 1 thread (the reference builds with OpenMP off)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards by "
+              "contiguous Morton element ranges with no exchange (see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from drifter_amd import _lib, synth
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    log("building 64x64x8 N=4 case ...")
+    case = synth.config2()
+    E, nd = case.E, case.nd
+    dof = E * nd
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+
+    s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
+    s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    U = torch.from_numpy(synth.to_soa(case.U, E, nd)).to(dev)
+    R = torch.empty_like(U)
+    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_SOA, stream),
+                       args.steps, args.warmup, stream)
+    gdof = dof * 5 / (ms * 1e-3) / 1e9
+    bytes_per = s64.rhs_bytes()
+    achieved = bytes_per / (ms * 1e-3) / 1e9
+    r64 = R.cpu().numpy()
+    log(f"FP64 RHS: {ms * 1e3:.1f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s")
+
+    # mixed precision (FP32 volume arithmetic and storage)
+    s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
+    s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    U32 = U.to(torch.float32)
+    R32 = torch.empty_like(U32)
+    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_SOA, stream),
+                         args.steps, args.warmup, stream)
+    err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
+    log(f"mixed RHS: {ms32 * 1e3:.1f} us, max rel L2 vs FP64 {max(err32):.2e}")
+
+    # RK4 steps: fused epilogue vs the reference's un-fused sequence
+    dt = 1e-3
+    Uw = U.clone()
+    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_SOA,
+                                                    True, stream), 10, 2, stream)
+    Uw = U.clone()
+    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_SOA,
+                                                    False, stream), 10, 2, stream)
+    Uw32 = U32.clone()
+    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_SOA,
+                                                    True, stream), 10, 2, stream)
+    log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
+        f"{ms_rk_m:.3f} ms")
+
+    cpu = None if args.no_cpu else cpu_baseline(case)
+    if cpu:
+        log(f"CPU baseline: {cpu['value']:.4f} GDOF/s ({cpu['sample']})")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_64x64x8_n4.traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    line = {
+        "metric": "DG RHS GDOF/s on 1 MI355X (FP64 vs mixed @ stated L2 err); % HBM roofline",
+        "value": round(gdof, 3),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "DG RHS, uniform 64x64x8 octree, N=4, tanh shelf, FP64, SoA in HBM",
+            "elements": E,
+            "order": 4,
+            "dof_per_field": dof,
+            "fields": 5,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per,
+        },
+        "cpu_baseline": cpu,
+        "extras": {
+            "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
+            "mixed_ms": round(ms32, 6),
+            "mixed_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
+            "mixed_hbm_gb_s": round(s32.rhs_bytes() / (ms32 * 1e-3) / 1e9, 1),
+            "rk4_step_ms_fp64_fused": round(ms_rk_f, 4),
+            "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
+            "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
+            "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,108 @@
+"""ctypes binding of libdrifter_hip.so (the C ABI declared in include/drifter_hip/drf.h).
+
+The library is the only compute path: there is no CPU fallback.  Loading fails
+loudly when the shared object has not been built.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdrifter_hip.so")
+
+DRF_OK = 0
+DRF_ERR_ARG = 1
+DRF_ERR_HIP = 2
+DRF_ERR_STATE = 3
+DRF_ERR_UNSUPPORTED = 4
+DRF_ERR_RANGE = 5
+
+LAYOUT_PACKED = 0
+LAYOUT_SOA = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+RK_EULER, RK_RK2, RK_SSPRK3, RK_RK4, RK_LSRK45, RK_DP45 = range(6)
+
+
+class DrfPhysics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("g", "nu_h", "nu_v", "kappa_v_T", "kappa_v_S", "rho_0", "alpha", "beta")]
+
+
+# name -> (restype, argtypes); every symbol declared in drf.h
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_L = C.c_int64
+_PD = C.POINTER(C.c_double)
+SIGNATURES = {
+    "drf_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "drf_destroy": (None, [_P]),
+    "drf_last_error": (C.c_char_p, []),
+    "drf_default_physics": (None, [C.POINTER(DrfPhysics)]),
+    "drf_set_physics": (_I, [_P, C.POINTER(DrfPhysics)]),
+    "drf_basis": (_I, [_I, _PD, _PD, _PD]),
+    "drf_initialize": (_I, [_P, _L, _PD, _PD, _PD, _D, _D, _D, _PD]),
+    "drf_rhs": (_I, [_P, _D, _PD, _PD]),
+    "drf_rhs_dev": (_I, [_P, _D, _P, _P, _I, _P]),
+    "drf_rk_step": (_I, [_P, _I, _D, _D, _PD]),
+    "drf_rk_step_dev": (_I, [_P, _I, _D, _D, _P, _I, _I, _P]),
+    "drf_max_wave_speed": (_I, [_P, _PD, _PD]),
+    "drf_max_wave_speed_dev": (_I, [_P, _P, _I, _PD, _P]),
+    "drf_max_wave_speed_derived_dev": (_I, [_P, _P, _P, _P, _L, _PD, _P]),
+    "drf_compute_dt": (_D, [_D, _D, _D, _D, _D]),
+    "drf_mode_split_step": (_I, [_P, _D, _I, _PD]),
+    "drf_mode_split_step_dev": (_I, [_P, _D, _I, _P, _P]),
+    "drf_rhs_bytes": (_D, [_P]),
+    "drf_num_elements": (_L, [_P]),
+    "drf_order": (_I, [_P]),
+}
+
+_lib = None
+
+
+class DrfError(RuntimeError):
+    """Raised for a non-zero drf_status (the reference raises C++ exceptions)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"drf status {status}: {msg}")
+        self.status = status
+
+
+def load():
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m drifter_amd.build` "
+            "(there is no CPU fallback)")
+    # torch ships its own libamdhip64 (same soname, libamdhip64.so.7).  Loading
+    # torch first makes the library bind to that already-loaded runtime, so a
+    # process that also uses torch for device buffers has one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != DRF_OK:
+        raise DrfError(status, load().drf_last_error().decode())
+    return status
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype.name == "float64" and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PD)

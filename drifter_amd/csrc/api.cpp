@@ -1,0 +1,674 @@
+// C ABI of libdrifter_hip: context lifecycle, right-hand side, time stepping,
+// CFL reduction and mode splitting.  Host code only; kernels live in *.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "drf_internal.hpp"
+#include "lincomb.hpp"
+
+namespace drf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+drf_status fail(drf_status code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+} // namespace drf
+
+using namespace drf;
+
+namespace {
+
+#define DRF_REQUIRE(cond, code, msg)                                                         \
+    do {                                                                                     \
+        if (!(cond))                                                                         \
+            return ::drf::fail((code), (msg));                                               \
+    } while (0)
+
+// Device entry points run on the caller's stream exactly as given; NULL is
+// HIP's null (default) stream, as everywhere in the HIP API.
+hipStream_t pick_stream(drf_ctx *c, void *s)
+{
+    (void)c;
+    return (hipStream_t)s;
+}
+
+size_t elem_size(const drf_ctx *c) { return c->precision == DRF_PREC_FP64 ? 8 : 4; }
+
+int64_t state_len(const drf_ctx *c) { return c->E * 5 * (int64_t)c->nd; }
+
+drf_status ensure_work(drf_ctx *c, size_t bytes)
+{
+    if (c->work_bytes >= bytes)
+        return DRF_OK;
+    if (c->d_work)
+        DRF_HIP_CHECK(hipFree(c->d_work));
+    c->d_work = nullptr;
+    c->work_bytes = 0;
+    DRF_HIP_CHECK(hipMalloc(&c->d_work, bytes));
+    c->work_bytes = bytes;
+    return DRF_OK;
+}
+
+drf_status ensure_red(drf_ctx *c, int64_t len)
+{
+    if (c->red_len >= len)
+        return DRF_OK;
+    if (c->d_red)
+        DRF_HIP_CHECK(hipFree(c->d_red));
+    c->d_red = nullptr;
+    c->red_len = 0;
+    DRF_HIP_CHECK(hipMalloc(&c->d_red, sizeof(double) * len));
+    c->red_len = len;
+    return DRF_OK;
+}
+
+drf_status check_ready(const drf_ctx *c)
+{
+    DRF_REQUIRE(c, DRF_ERR_ARG, "null context");
+    DRF_REQUIRE(c->E > 0 && c->d_h, DRF_ERR_STATE, "drf_initialize has not been called");
+    return DRF_OK;
+}
+
+template <class T> View<T> view(void *p, int layout, const drf_ctx *c)
+{
+    View<T> v{(T *)p, 0, 0};
+    layout_strides(layout, c->E, c->nd, 5, v.fs, v.es);
+    return v;
+}
+
+template <class T> View<const T> cview(const void *p, int layout, const drf_ctx *c)
+{
+    View<const T> v{(const T *)p, 0, 0};
+    layout_strides(layout, c->E, c->nd, 5, v.fs, v.es);
+    return v;
+}
+
+// One volume-kernel launch at the context's precision.
+drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode, void *R,
+                  const void *U0, void *Y, void *A, const void *X, double c0, double c1,
+                  double c2, double c3, hipStream_t s)
+{
+    hipError_t err;
+    if (c->precision == DRF_PREC_FP64) {
+        Epi<double> epi{};
+        epi.mode = mode;
+        epi.acc_mode = acc_mode;
+        epi.R = view<double>(R, layout, c);
+        epi.U0 = cview<double>(U0, layout, c);
+        epi.Y = view<double>(Y, layout, c);
+        epi.A = view<double>(A, layout, c);
+        epi.X = cview<double>(X, layout, c);
+        epi.c0 = c0;
+        epi.c1 = c1;
+        epi.c2 = c2;
+        epi.c3 = c3;
+        VolGeom<double> geo{c->E, c->d_h, c->d_f, c->f0};
+        err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
+                                 s);
+    } else {
+        Epi<float> epi{};
+        epi.mode = mode;
+        epi.acc_mode = acc_mode;
+        epi.R = view<float>(R, layout, c);
+        epi.U0 = cview<float>(U0, layout, c);
+        epi.Y = view<float>(Y, layout, c);
+        epi.A = view<float>(A, layout, c);
+        epi.X = cview<float>(X, layout, c);
+        epi.c0 = c0;
+        epi.c1 = c1;
+        epi.c2 = c2;
+        epi.c3 = c3;
+        VolGeom<float> geo{c->E, c->d_h32, c->d_f32, c->f0};
+        err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
+                                 s);
+    }
+    if (err != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("volume kernel launch: ") + hipGetErrorString(err));
+    return DRF_OK;
+}
+
+drf_status rhs_into(drf_ctx *c, const void *U, void *R, int layout, hipStream_t s)
+{
+    return volume(c, U, layout, EPI_RHS, 0, R, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0, s);
+}
+
+// Un-fused stepping: the reference's sequence of RHS evaluations and vector
+// updates (FP64 only).
+drf_status rk_unfused(drf_ctx *c, int scheme, double dt, double *U, int layout, hipStream_t s)
+{
+    DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "un-fused stepping is the FP64 baseline only");
+    const int64_t len = state_len(c);
+    drf_status st = ensure_work(c, sizeof(double) * len * 7);
+    if (st)
+        return st;
+    double *w = (double *)c->d_work;
+    double *k[6] = {w, w + len, w + 2 * len, w + 3 * len, w + 4 * len, w + 5 * len};
+    double *Ut = w + 6 * len;
+    auto V = [&](double *p) { return view<double>(p, layout, c); };
+    auto CV = [&](const double *p) { return cview<double>(p, layout, c); };
+    auto lin = [&](double *Y, const double *X0, double alpha, const double *X1, double beta,
+                   double gamma, std::initializer_list<std::pair<const double *, double>> ks) {
+        LinComb lc{};
+        lc.Y = V(Y);
+        lc.X0 = CV(X0);
+        lc.X1 = CV(X1);
+        lc.alpha = alpha;
+        lc.beta = beta;
+        lc.gamma = gamma;
+        lc.nk = 0;
+        for (auto &kc : ks) {
+            lc.K[lc.nk] = CV(kc.first);
+            lc.c[lc.nk] = kc.second;
+            ++lc.nk;
+        }
+        hipError_t e = launch_lincomb(c->E, c->nd, lc, s);
+        return e == hipSuccess ? DRF_OK : fail(DRF_ERR_HIP, hipGetErrorString(e));
+    };
+#define TRY(x)                                                                               \
+    do {                                                                                     \
+        drf_status t_ = (x);                                                                 \
+        if (t_)                                                                              \
+            return t_;                                                                       \
+    } while (0)
+    switch (scheme) {
+    case DRF_RK_EULER: // :14-18
+        TRY(rhs_into(c, U, k[0], layout, s));
+        TRY(lin(U, U, 1.0, nullptr, 1.0, dt, {{k[0], 1.0}}));
+        break;
+    case DRF_RK_RK2: // :33-47
+        TRY(rhs_into(c, U, k[0], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, 0.5 * dt, {{k[0], 1.0}}));
+        TRY(rhs_into(c, Ut, k[1], layout, s));
+        TRY(lin(U, U, 1.0, nullptr, 1.0, dt, {{k[1], 1.0}}));
+        break;
+    case DRF_RK_SSPRK3: // :64-80 (U1 in Ut, U2 in k[1])
+        TRY(rhs_into(c, U, k[0], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt, {{k[0], 1.0}}));
+        TRY(rhs_into(c, Ut, k[0], layout, s));
+        TRY(lin(k[1], U, 0.75, Ut, 0.25, dt, {{k[0], 1.0}}));
+        TRY(rhs_into(c, k[1], k[0], layout, s));
+        TRY(lin(U, U, 1.0 / 3.0, k[1], 2.0 / 3.0, dt, {{k[0], 1.0}}));
+        break;
+    case DRF_RK_RK4: // :88-112
+        TRY(rhs_into(c, U, k[0], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, 0.5 * dt, {{k[0], 1.0}}));
+        TRY(rhs_into(c, Ut, k[1], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, 0.5 * dt, {{k[1], 1.0}}));
+        TRY(rhs_into(c, Ut, k[2], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt, {{k[2], 1.0}}));
+        TRY(rhs_into(c, Ut, k[3], layout, s));
+        TRY(lin(U, U, 1.0, nullptr, 1.0, dt / 6.0,
+                {{k[0], 1.0}, {k[1], 2.0}, {k[2], 2.0}, {k[3], 1.0}}));
+        break;
+    case DRF_RK_LSRK45: { // :126-143
+        static const double A[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471,
+                                    -1.514183444257};
+        static const double B[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
+                                    0.6994504559488, 0.1530572479681};
+        // The reference zeroes the residual first; A[0] = 0 so stage 0 is
+        // res = 0 * 0 + dt k = dt k, written directly (no memset is recorded).
+        double *res = k[1];
+        for (int st5 = 0; st5 < 5; ++st5) {
+            TRY(rhs_into(c, U, k[0], layout, s));
+            TRY(lin(res, st5 == 0 ? nullptr : res, A[st5], nullptr, 1.0, dt, {{k[0], 1.0}}));
+            TRY(lin(U, U, 1.0, nullptr, 1.0, B[st5], {{res, 1.0}}));
+        }
+        break;
+    }
+    case DRF_RK_DP45: // :151-189
+        TRY(rhs_into(c, U, k[0], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt / 5.0, {{k[0], 1.0}}));
+        TRY(rhs_into(c, Ut, k[1], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt, {{k[0], 3.0 / 40.0}, {k[1], 9.0 / 40.0}}));
+        TRY(rhs_into(c, Ut, k[2], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt,
+                {{k[0], 44.0 / 45.0}, {k[1], -56.0 / 15.0}, {k[2], 32.0 / 9.0}}));
+        TRY(rhs_into(c, Ut, k[3], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt,
+                {{k[0], 19372.0 / 6561.0},
+                 {k[1], -25360.0 / 2187.0},
+                 {k[2], 64448.0 / 6561.0},
+                 {k[3], -212.0 / 729.0}}));
+        TRY(rhs_into(c, Ut, k[4], layout, s));
+        TRY(lin(Ut, U, 1.0, nullptr, 1.0, dt,
+                {{k[0], 9017.0 / 3168.0},
+                 {k[1], -355.0 / 33.0},
+                 {k[2], 46732.0 / 5247.0},
+                 {k[3], 49.0 / 176.0},
+                 {k[4], -5103.0 / 18656.0}}));
+        TRY(rhs_into(c, Ut, k[5], layout, s));
+        TRY(lin(U, U, 1.0, nullptr, 1.0, dt,
+                {{k[0], 35.0 / 384.0},
+                 {k[2], 500.0 / 1113.0},
+                 {k[3], 125.0 / 192.0},
+                 {k[4], -2187.0 / 6784.0},
+                 {k[5], 11.0 / 84.0}}));
+        break;
+    default:
+        return fail(DRF_ERR_ARG, "unknown Runge-Kutta scheme " + std::to_string(scheme));
+    }
+    return DRF_OK;
+}
+
+// Fused stepping: each stage's vector update runs in the RHS kernel epilogue.
+drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipStream_t s)
+{
+    if (scheme == DRF_RK_DP45)
+        return rk_unfused(c, scheme, dt, (double *)U, layout, s);
+    const int64_t len = state_len(c);
+    const size_t es = elem_size(c);
+    drf_status st = ensure_work(c, es * len * 2);
+    if (st)
+        return st;
+    char *w = (char *)c->d_work;
+    void *Ut = w, *A = w + es * len;
+    switch (scheme) {
+    case DRF_RK_EULER:
+        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, U, nullptr, nullptr, 0, dt, 0, 0, s));
+        break;
+    case DRF_RK_RK2:
+        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, Ut, nullptr, nullptr, 0, 0.5 * dt, 0,
+                   0, s));
+        TRY(volume(c, Ut, layout, EPI_STAGE, 0, nullptr, U, U, nullptr, nullptr, 0, dt, 0, 0, s));
+        break;
+    case DRF_RK_SSPRK3:
+        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, Ut, nullptr, nullptr, 0, dt, 0, 0, s));
+        TRY(volume(c, Ut, layout, EPI_COMB, 0, nullptr, U, Ut, nullptr, Ut, 0.75, 0.25, dt, 0, s));
+        TRY(volume(c, Ut, layout, EPI_COMB, 0, nullptr, U, U, nullptr, Ut, 1.0 / 3.0, 2.0 / 3.0,
+                   dt, 0, s));
+        break;
+    case DRF_RK_RK4:
+        TRY(volume(c, U, layout, EPI_STAGE, 1, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 0, 0, s));
+        TRY(volume(c, Ut, layout, EPI_STAGE, 2, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 2.0, 0,
+                   s));
+        TRY(volume(c, Ut, layout, EPI_STAGE, 2, nullptr, U, Ut, A, nullptr, 0, dt, 2.0, 0, s));
+        TRY(volume(c, Ut, layout, EPI_RK4FIN, 0, nullptr, U, U, A, nullptr, 0, dt / 6.0, 0, 0,
+                   s));
+        break;
+    case DRF_RK_LSRK45: {
+        static const double Ac[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471,
+                                     -1.514183444257};
+        static const double Bc[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
+                                     0.6994504559488, 0.1530572479681};
+        for (int k = 0; k < 5; ++k)
+            TRY(volume(c, U, layout, EPI_LSRK, k == 0 ? 1 : 2, nullptr, nullptr, U, A, U, 0, dt,
+                       Ac[k], Bc[k], s));
+        break;
+    }
+    default:
+        return fail(DRF_ERR_ARG, "unknown Runge-Kutta scheme " + std::to_string(scheme));
+    }
+    return DRF_OK;
+}
+
+drf_status wave_speed(drf_ctx *c, const void *U, int layout, double *out, hipStream_t s)
+{
+    DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "wave speed on the FP64 state only");
+    const int64_t npart = std::min<int64_t>(2048, (c->E + 3) / 4);
+    drf_status st = ensure_red(c, npart + 1);
+    if (st)
+        return st;
+    hipError_t e = launch_wave_speed(c->order, c->E, cview<double>(U, layout, c), c->d_h,
+                                     c->d_red + 1, npart, s);
+    if (e == hipSuccess)
+        e = launch_max_reduce(c->d_red + 1, npart, c->d_red, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, c->d_red, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("wave speed: ") + hipGetErrorString(e));
+    return DRF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void drf_default_physics(drf_physics *p)
+{
+    if (!p)
+        return;
+    p->g = 9.81;
+    p->nu_h = 1.0;
+    p->nu_v = 1e-4;
+    p->kappa_v_T = 1e-5;
+    p->kappa_v_S = 1e-5;
+    p->rho_0 = 1025.0;
+    p->alpha = 2e-4;
+    p->beta = 7.6e-4;
+}
+
+const char *drf_last_error(void) { return g_last_error.c_str(); }
+
+drf_status drf_basis(int order, double *nodes, double *weights, double *D)
+{
+    DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_ARG,
+                "order must be in [1, 8], got " + std::to_string(order));
+    double x[kMaxN], w[kMaxN], Dm[kMaxN * kMaxN];
+    if (lgl_nodes(order + 1, x, w))
+        return fail(DRF_ERR_ARG, "LGL requires at least 2 nodes");
+    derivative_matrix(order + 1, x, Dm);
+    const int n = order + 1;
+    if (nodes)
+        std::memcpy(nodes, x, sizeof(double) * n);
+    if (weights)
+        std::memcpy(weights, w, sizeof(double) * n);
+    if (D)
+        std::memcpy(D, Dm, sizeof(double) * n * n);
+    return DRF_OK;
+}
+
+drf_status drf_create(int order, int precision, int device, drf_ctx **out)
+{
+    DRF_REQUIRE(out, DRF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_UNSUPPORTED,
+                "order must be in [1, 8], got " + std::to_string(order));
+    DRF_REQUIRE(precision == DRF_PREC_FP64 || precision == DRF_PREC_MIXED, DRF_ERR_ARG,
+                "unknown precision " + std::to_string(precision));
+    int ndev = 0;
+    DRF_HIP_CHECK(hipGetDeviceCount(&ndev));
+    DRF_REQUIRE(device >= 0 && device < ndev, DRF_ERR_ARG,
+                "device " + std::to_string(device) + " not present");
+    DRF_HIP_CHECK(hipSetDevice(device));
+    drf_ctx *c = new drf_ctx();
+    c->order = order;
+    c->n = order + 1;
+    c->nh = c->n * c->n;
+    c->nd = c->nh * c->n;
+    c->precision = precision;
+    c->device = device;
+    double x[kMaxN], w[kMaxN];
+    lgl_nodes(c->n, x, w);
+    derivative_matrix(c->n, x, c->ops.D);
+    for (int k = 0; k < c->n; ++k)
+        c->ops.sw[k] = 0.5 * w[k]; // sigma_weights_, primitive_equations.cpp:116-121
+    drf_physics p;
+    drf_default_physics(&p);
+    c->phys = Phys{p.g, p.nu_h, p.nu_v, p.kappa_v_T, p.kappa_v_S, p.rho_0, p.alpha, p.beta};
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(DRF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return DRF_OK;
+}
+
+void drf_destroy(drf_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_h);
+    (void)hipFree(c->d_f);
+    (void)hipFree(c->d_h32);
+    (void)hipFree(c->d_f32);
+    (void)hipFree(c->d_work);
+    (void)hipFree(c->d_red);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+drf_status drf_set_physics(drf_ctx *c, const drf_physics *p)
+{
+    DRF_REQUIRE(c && p, DRF_ERR_ARG, "null argument");
+    c->phys = Phys{p->g, p->nu_h, p->nu_v, p->kappa_v_T, p->kappa_v_S, p->rho_0, p->alpha, p->beta};
+    return DRF_OK;
+}
+
+drf_status drf_initialize(drf_ctx *c, int64_t E, const double *h, const double *dh_dx,
+                          const double *dh_dy, double f0, double beta, double y0,
+                          const double *y)
+{
+    (void)dh_dx;
+    (void)dh_dy;
+    DRF_REQUIRE(c, DRF_ERR_ARG, "null context");
+    DRF_REQUIRE(E > 0, DRF_ERR_ARG, "number of elements must be positive");
+    DRF_REQUIRE(h, DRF_ERR_ARG, "bathymetry is required");
+    DRF_REQUIRE(y || beta == 0.0, DRF_ERR_ARG, "y_positions are required when beta != 0");
+    DRF_HIP_CHECK(hipSetDevice(c->device));
+    const int64_t len = E * c->nd;
+    (void)hipFree(c->d_h);
+    (void)hipFree(c->d_f);
+    (void)hipFree(c->d_h32);
+    (void)hipFree(c->d_f32);
+    c->d_h = c->d_f = nullptr;
+    c->d_h32 = c->d_f32 = nullptr;
+    c->E = E;
+    c->f0 = f0;
+    DRF_HIP_CHECK(hipMalloc(&c->d_h, sizeof(double) * len));
+    DRF_HIP_CHECK(hipMemcpy(c->d_h, h, sizeof(double) * len, hipMemcpyHostToDevice));
+    std::vector<double> f;
+    if (beta != 0.0) {
+        // CoriolisParameter::compute, primitive_equations.hpp:65-70
+        f.resize(len);
+        for (int64_t i = 0; i < len; ++i)
+            f[i] = f0 + beta * (y[i] - y0);
+        DRF_HIP_CHECK(hipMalloc(&c->d_f, sizeof(double) * len));
+        DRF_HIP_CHECK(hipMemcpy(c->d_f, f.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    }
+    if (c->precision == DRF_PREC_MIXED) {
+        std::vector<float> tmp(len);
+        for (int64_t i = 0; i < len; ++i)
+            tmp[i] = (float)h[i];
+        DRF_HIP_CHECK(hipMalloc(&c->d_h32, sizeof(float) * len));
+        DRF_HIP_CHECK(hipMemcpy(c->d_h32, tmp.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+        if (!f.empty()) {
+            for (int64_t i = 0; i < len; ++i)
+                tmp[i] = (float)f[i];
+            DRF_HIP_CHECK(hipMalloc(&c->d_f32, sizeof(float) * len));
+            DRF_HIP_CHECK(
+                hipMemcpy(c->d_f32, tmp.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+        }
+    }
+    return DRF_OK;
+}
+
+drf_status drf_rhs_dev(drf_ctx *c, double t, const void *U, void *dUdt, int layout, void *stream)
+{
+    (void)t; // the reference RHS does not depend on time
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && dUdt, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA, DRF_ERR_ARG,
+                "unknown layout");
+    return rhs_into(c, U, dUdt, layout, pick_stream(c, stream));
+}
+
+drf_status drf_rhs(drf_ctx *c, double t, const double *U, double *dUdt)
+{
+    (void)t;
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && dUdt, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "host-pointer RHS is provided for FP64 contexts");
+    const int64_t len = state_len(c);
+    st = ensure_work(c, sizeof(double) * len * 2);
+    if (st)
+        return st;
+    double *dU = (double *)c->d_work, *dR = dU + len;
+    hipStream_t s = c->stream;
+    DRF_HIP_CHECK(hipMemcpyAsync(dU, U, sizeof(double) * len, hipMemcpyHostToDevice, s));
+    st = rhs_into(c, dU, dR, DRF_LAYOUT_PACKED, s);
+    if (st)
+        return st;
+    DRF_HIP_CHECK(hipMemcpyAsync(dUdt, dR, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    DRF_HIP_CHECK(hipStreamSynchronize(s));
+    return DRF_OK;
+}
+
+drf_status drf_rk_step_dev(drf_ctx *c, int scheme, double t, double dt, void *U, int layout,
+                           int fused, void *stream)
+{
+    (void)t;
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA, DRF_ERR_ARG,
+                "unknown layout");
+    DRF_REQUIRE(scheme >= DRF_RK_EULER && scheme <= DRF_RK_DP45, DRF_ERR_ARG,
+                "unknown Runge-Kutta scheme " + std::to_string(scheme));
+    hipStream_t s = pick_stream(c, stream);
+    if (fused)
+        return rk_fused(c, scheme, dt, U, layout, s);
+    return rk_unfused(c, scheme, dt, (double *)U, layout, s);
+}
+
+drf_status drf_rk_step(drf_ctx *c, int scheme, double t, double dt, double *U)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "host-pointer stepping is provided for FP64 contexts");
+    const int64_t len = state_len(c);
+    double *dU = nullptr;
+    DRF_HIP_CHECK(hipMalloc(&dU, sizeof(double) * len));
+    hipStream_t s = c->stream;
+    hipError_t e = hipMemcpyAsync(dU, U, sizeof(double) * len, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        st = drf_rk_step_dev(c, scheme, t, dt, dU, DRF_LAYOUT_PACKED, 1, s);
+        if (st == DRF_OK) {
+            e = hipMemcpyAsync(U, dU, sizeof(double) * len, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(s);
+        }
+    }
+    (void)hipFree(dU);
+    if (st)
+        return st;
+    if (e != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("drf_rk_step: ") + hipGetErrorString(e));
+    return DRF_OK;
+}
+
+drf_status drf_max_wave_speed_dev(drf_ctx *c, const void *U, int layout, double *c_max,
+                                  void *stream)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && c_max, DRF_ERR_ARG, "null argument");
+    return wave_speed(c, U, layout, c_max, pick_stream(c, stream));
+}
+
+drf_status drf_max_wave_speed(drf_ctx *c, const double *U, double *c_max)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && c_max, DRF_ERR_ARG, "null argument");
+    const int64_t len = state_len(c);
+    st = ensure_work(c, sizeof(double) * len);
+    if (st)
+        return st;
+    DRF_HIP_CHECK(
+        hipMemcpyAsync(c->d_work, U, sizeof(double) * len, hipMemcpyHostToDevice, c->stream));
+    return wave_speed(c, c->d_work, DRF_LAYOUT_PACKED, c_max, c->stream);
+}
+
+drf_status drf_max_wave_speed_derived_dev(drf_ctx *c, const void *H, const void *u,
+                                          const void *v, int64_t es, double *c_max, void *stream)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(H && u && v && c_max, DRF_ERR_ARG, "null argument");
+    hipStream_t s = pick_stream(c, stream);
+    const int64_t npart = std::min<int64_t>(2048, (c->E + 3) / 4);
+    st = ensure_red(c, npart + 1);
+    if (st)
+        return st;
+    hipError_t e = launch_wave_speed_derived(c->order, c->E, (const double *)H, (const double *)u,
+                                             (const double *)v, es, c->d_red + 1, npart, s);
+    if (e == hipSuccess)
+        e = launch_max_reduce(c->d_red + 1, npart, c->d_red, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c_max, c->d_red, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("wave speed: ") + hipGetErrorString(e));
+    return DRF_OK;
+}
+
+double drf_compute_dt(double cmax, double dx_min, double cfl, double dt_min, double dt_max)
+{
+    // AdaptiveTimeController::compute_dt, src/solver/time_stepper.cpp:261-268
+    if (cmax < 1e-14)
+        return dt_max;
+    return std::clamp(cfl * dx_min / cmax, dt_min, dt_max);
+}
+
+drf_status drf_mode_split_step_dev(drf_ctx *c, double dt3, int nsub, void *full, void *stream)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(full, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(nsub >= 1, DRF_ERR_ARG, "subcycles must be >= 1");
+    hipError_t e = launch_mode_split(c->order, c->E, dt3, nsub, (double *)full, c->d_h, c->d_f,
+                                     c->f0, c->ops, c->phys, pick_stream(c, stream));
+    if (e != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("mode split launch: ") + hipGetErrorString(e));
+    return DRF_OK;
+}
+
+drf_status drf_mode_split_step(drf_ctx *c, double dt3, int nsub, double *full)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(full, DRF_ERR_ARG, "null state pointer");
+    const int64_t len = c->E * 12 * (int64_t)c->nd;
+    st = ensure_work(c, sizeof(double) * len);
+    if (st)
+        return st;
+    hipStream_t s = c->stream;
+    DRF_HIP_CHECK(hipMemcpyAsync(c->d_work, full, sizeof(double) * len, hipMemcpyHostToDevice, s));
+    st = drf_mode_split_step_dev(c, dt3, nsub, c->d_work, s);
+    if (st)
+        return st;
+    DRF_HIP_CHECK(hipMemcpyAsync(full, c->d_work, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    DRF_HIP_CHECK(hipStreamSynchronize(s));
+    return DRF_OK;
+}
+
+double drf_rhs_bytes(const drf_ctx *c)
+{
+    if (!c)
+        return 0.0;
+    // 5 state reads + 1 bathymetry read + 5 tendency writes per node (+ Coriolis
+    // when it varies per node).
+    const double s = (c->precision == DRF_PREC_FP64) ? 8.0 : 4.0;
+    const double arrays = 11.0 + ((c->d_f || c->d_f32) ? 1.0 : 0.0);
+    return (double)c->E * c->nd * arrays * s;
+}
+
+int64_t drf_num_elements(const drf_ctx *c) { return c ? c->E : 0; }
+int drf_order(const drf_ctx *c) { return c ? c->order : 0; }
+
+} // extern "C"

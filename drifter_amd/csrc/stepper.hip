@@ -1,0 +1,160 @@
+// Streaming kernels around the right-hand side: the un-fused Runge-Kutta
+// vector update (the reference's own sequence, src/solver/time_stepper.cpp:12-201,
+// kept as the "plain" baseline) and the CFL wave-speed reduction
+// (SimulationDriver::compute_max_wave_speed, src/solver/simulation_driver.cpp:153-168).
+#include "drf_internal.hpp"
+
+namespace drf {
+namespace {
+
+// Y = X + a K over the 5 state fields; one thread per node, grid-stride.
+__global__ void __launch_bounds__(256)
+    axpy_fields_kernel(int64_t E, int nd, View<double> Y, View<const double> X, double a,
+                       View<const double> K)
+{
+    const int64_t total = E * nd;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t / nd, d = t - e * nd;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const double x = X.p[f * X.fs + e * X.es + d];
+            const double k = K.p[f * K.fs + e * K.es + d];
+            Y.p[f * Y.fs + e * Y.es + d] = x + a * k;
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_max(double a, double b) { return a > b ? a : b; }
+
+// Per element: sqrt(9.81 * max H) + max(max|u|, max|v|) on the derived state
+// of U (update_derived, src/physics/primitive_equations.cpp:21-28).  One wave
+// per element; partial[block] = max over the block's elements.
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+    wave_speed_kernel(int64_t E, int nd, View<const double> U, const double *h, double *partial)
+{
+    __shared__ double red[WAVES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double cmax = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * WAVES + w; e < E; e += (int64_t)gridDim.x * WAVES) {
+        double hm = -INFINITY, um = 0.0, vm = 0.0;
+        for (int d = lane; d < nd; d += 64) {
+            const double eta = U.p[2 * U.fs + e * U.es + d];
+            const double H = eta + h[e * nd + d];
+            const double hi = (H > 1e-10) ? 1.0 / H : 0.0;
+            const double u = U.p[0 * U.fs + e * U.es + d] * hi;
+            const double v = U.p[1 * U.fs + e * U.es + d] * hi;
+            hm = wave_max(hm, H);
+            um = wave_max(um, fabs(u));
+            vm = wave_max(vm, fabs(v));
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            hm = wave_max(hm, __shfl_xor(hm, o));
+            um = wave_max(um, __shfl_xor(um, o));
+            vm = wave_max(vm, __shfl_xor(vm, o));
+        }
+        const double ce = sqrt(9.81 * hm) + wave_max(um, vm);
+        cmax = wave_max(cmax, ce);
+    }
+    if (lane == 0)
+        red[w] = cmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int i = 1; i < WAVES; ++i)
+            m = wave_max(m, red[i]);
+        partial[blockIdx.x] = m;
+    }
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+    wave_speed_derived_kernel(int64_t E, int nd, const double *H, const double *u,
+                              const double *v, int64_t es, double *partial)
+{
+    __shared__ double red[WAVES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double cmax = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * WAVES + w; e < E; e += (int64_t)gridDim.x * WAVES) {
+        double hm = -INFINITY, um = 0.0, vm = 0.0;
+        for (int d = lane; d < nd; d += 64) {
+            hm = wave_max(hm, H[e * es + d]);
+            um = wave_max(um, fabs(u[e * es + d]));
+            vm = wave_max(vm, fabs(v[e * es + d]));
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            hm = wave_max(hm, __shfl_xor(hm, o));
+            um = wave_max(um, __shfl_xor(um, o));
+            vm = wave_max(vm, __shfl_xor(vm, o));
+        }
+        cmax = wave_max(cmax, sqrt(9.81 * hm) + wave_max(um, vm));
+    }
+    if (lane == 0)
+        red[w] = cmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int i = 1; i < WAVES; ++i)
+            m = wave_max(m, red[i]);
+        partial[blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(256) max_reduce_kernel(const double *p, int64_t np, double *out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < np; i += 256)
+        m = wave_max(m, p[i]);
+    for (int o = 32; o > 0; o >>= 1)
+        m = wave_max(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[0] = wave_max(wave_max(red[0], red[1]), wave_max(red[2], red[3]));
+}
+
+} // namespace
+
+hipError_t launch_axpy_fields(int64_t E, int nd, View<double> Y, View<const double> X, double a,
+                              View<const double> K, hipStream_t s)
+{
+    const int64_t total = E * nd;
+    if (total <= 0)
+        return hipSuccess;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    hipLaunchKernelGGL(axpy_fields_kernel, dim3((unsigned)blocks), dim3(256), 0, s, E, nd, Y, X,
+                       a, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, const double *h,
+                             double *partial, int64_t npart, hipStream_t s)
+{
+    const int n = order + 1, nd = n * n * n;
+    hipLaunchKernelGGL(wave_speed_kernel<4>, dim3((unsigned)npart), dim3(256), 0, s, E, nd, U, h,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_speed_derived(int order, int64_t E, const double *H, const double *u,
+                                     const double *v, int64_t es, double *partial,
+                                     int64_t npart, hipStream_t s)
+{
+    const int n = order + 1, nd = n * n * n;
+    hipLaunchKernelGGL(wave_speed_derived_kernel<4>, dim3((unsigned)npart), dim3(256), 0, s, E,
+                       nd, H, u, v, es, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_reduce(const double *partial, int64_t npart, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, s, partial, npart, out);
+    return hipGetLastError();
+}
+
+} // namespace drf

@@ -1,0 +1,155 @@
+/*
+ * drf.h -- C ABI of the MI355X (gfx950) DG right-hand side for DRIFTER.
+ *
+ * Plain pointers and sizes only: no HIP, torch or Eigen types appear in the
+ * signatures.  Device pointers are passed as void*; a HIP stream as void*
+ * (a hipStream_t; NULL is HIP's null stream).  Device (_dev) entry points are
+ * asynchronous on that stream unless they return a host value; host-pointer
+ * entry points use the context's own stream and return when done.
+ *
+ * Layouts (d = i + n(j + n k), xi fastest, include/core/types.hpp:83):
+ *   DRF_LAYOUT_PACKED  U[e][field][d]   -- the reference's packed vector
+ *                                          (src/solver/simulation_driver.cpp:199-213)
+ *   DRF_LAYOUT_SOA     U[field][e][d]   -- structure of arrays
+ *   fields: 0 Hu, 1 Hv, 2 eta, 3 HT, 4 HS
+ *   full state (mode splitting): [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d]
+ *
+ * Errors: every call returns a drf_status (0 = DRF_OK); the message of the
+ * last failure on the calling thread is returned by drf_last_error().  The C++
+ * facade (primitive_equations_gpu.hpp) turns non-zero statuses into
+ * std::runtime_error, as the reference reports errors by exception.
+ *
+ * Threading: one context per host thread (or per stream); calls on one
+ * context are stream-ordered and not re-entrant.
+ */
+#ifndef DRIFTER_HIP_DRF_H
+#define DRIFTER_HIP_DRF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct drf_ctx drf_ctx;
+typedef int drf_status;
+
+#define DRF_OK 0
+#define DRF_ERR_ARG 1         /* invalid argument (std::invalid_argument) */
+#define DRF_ERR_HIP 2         /* HIP runtime failure */
+#define DRF_ERR_STATE 3       /* call out of order, e.g. RHS before initialize */
+#define DRF_ERR_UNSUPPORTED 4 /* order or option not built */
+#define DRF_ERR_RANGE 5       /* index out of range (std::out_of_range) */
+
+#define DRF_LAYOUT_PACKED 0
+#define DRF_LAYOUT_SOA 1
+
+#define DRF_PREC_FP64 0  /* FP64 storage and arithmetic (parity path) */
+#define DRF_PREC_MIXED 1 /* FP32 volume arithmetic and storage; FP64 column integrals */
+
+/* Time-stepping schemes of src/solver/time_stepper.cpp */
+#define DRF_RK_EULER 0  /* ForwardEuler        :12-25   */
+#define DRF_RK_RK2 1    /* RK2Midpoint         :31-54   */
+#define DRF_RK_SSPRK3 2 /* SSPRK3              :64-80   */
+#define DRF_RK_RK4 3    /* RK4Classic          :86-119  */
+#define DRF_RK_LSRK45 4 /* LSRK45              :126-143 */
+#define DRF_RK_DP45 5   /* RK45DormandPrince   :149-201 */
+
+/* OceanConstants + DiffusivityParams, include/physics/primitive_equations.hpp:41-49,106-117 */
+typedef struct {
+    double g;         /* 9.81     */
+    double nu_h;      /* 1.0      */
+    double nu_v;      /* 1e-4     */
+    double kappa_v_T; /* 1e-5     */
+    double kappa_v_S; /* 1e-5     */
+    double rho_0;     /* 1025.0   */
+    double alpha;     /* 2e-4     */
+    double beta;      /* 7.6e-4   */
+} drf_physics;
+
+/* ------------------------------------------------------------------ */
+/* Lifecycle                                                           */
+/* ------------------------------------------------------------------ */
+
+/* Replaces PrimitiveEquationsSolver(int order, const OceanConstants&)
+ * (include/physics/primitive_equations.hpp:222).  precision: DRF_PREC_*. */
+drf_status drf_create(int order, int precision, int device, drf_ctx **out);
+void drf_destroy(drf_ctx *ctx);
+const char *drf_last_error(void);
+void drf_default_physics(drf_physics *p);
+drf_status drf_set_physics(drf_ctx *ctx, const drf_physics *p);
+/* 1D LGL nodes, weights and derivative matrix (n*n row-major) used by the
+ * kernels (HexahedronBasis, src/dg/basis_hexahedron.cpp:257-264). */
+drf_status drf_basis(int order, double *nodes, double *weights, double *D);
+
+/* Replaces PrimitiveEquationsSolver::initialize
+ * (include/physics/primitive_equations.hpp:229-231): bathymetry h at every
+ * DOF (E*ndof, element-major); dh_dx/dh_dy are accepted and ignored as in
+ * the reference (stored but unused by the RHS).  Coriolis: the beta-plane
+ * f = f0 + beta (y - y0) (CoriolisParameter, :58-62) at the DOF positions
+ * y_positions (E*ndof); y_positions may be NULL only when beta == 0. */
+drf_status drf_initialize(drf_ctx *ctx, int64_t n_elem, const double *h, const double *dh_dx,
+                          const double *dh_dy, double f0, double beta, double y0,
+                          const double *y_positions);
+
+/* ------------------------------------------------------------------ */
+/* Right-hand side                                                     */
+/* ------------------------------------------------------------------ */
+
+/* RHSFunction drop-in (include/solver/time_stepper.hpp:10): host packed U
+ * and dUdt of length E*5*ndof, as SimulationDriver::compute_rhs
+ * (src/solver/simulation_driver.cpp:170-197) produces it. */
+drf_status drf_rhs(drf_ctx *ctx, double t, const double *U, double *dUdt);
+
+/* Device-resident variant: U and dUdt are device pointers in `layout`,
+ * element type double (FP64 context) or float (mixed context). */
+drf_status drf_rhs_dev(drf_ctx *ctx, double t, const void *U, void *dUdt, int layout,
+                       void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Time stepping (TimeStepper::step, include/solver/time_stepper.hpp:18) */
+/* ------------------------------------------------------------------ */
+
+/* One step of `scheme` on the host packed state U (in/out). */
+drf_status drf_rk_step(drf_ctx *ctx, int scheme, double t, double dt, double *U);
+/* Device-resident state; fused != 0 fuses the stage updates into the RHS
+ * kernel epilogue, fused == 0 runs the reference's sequence (RHS, then
+ * separate vector updates). */
+drf_status drf_rk_step_dev(drf_ctx *ctx, int scheme, double t, double dt, void *U, int layout,
+                           int fused, void *stream);
+
+/* SimulationDriver::compute_max_wave_speed (src/solver/simulation_driver.cpp:153-168)
+ * evaluated on the derived state of U (PrimitiveState::update_derived). */
+drf_status drf_max_wave_speed(drf_ctx *ctx, const double *U, double *c_max);
+drf_status drf_max_wave_speed_dev(drf_ctx *ctx, const void *U, int layout, double *c_max,
+                                  void *stream);
+/* Same, on derived H, u, v arrays (device, E*ndof element-major each). */
+drf_status drf_max_wave_speed_derived_dev(drf_ctx *ctx, const void *H, const void *u,
+                                          const void *v, int64_t elem_stride, double *c_max,
+                                          void *stream);
+/* AdaptiveTimeController::compute_dt (src/solver/time_stepper.cpp:261-268). */
+double drf_compute_dt(double max_wave_speed, double dx_min, double cfl, double dt_min,
+                      double dt_max);
+
+/* ------------------------------------------------------------------ */
+/* Mode splitting (ModeSplittingSolver::step, include/physics/mode_splitting.hpp:241) */
+/* ------------------------------------------------------------------ */
+
+/* full: E*12*ndof FP64, [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d], in/out. */
+drf_status drf_mode_split_step(drf_ctx *ctx, double dt_3d, int n_sub, double *full);
+drf_status drf_mode_split_step_dev(drf_ctx *ctx, double dt_3d, int n_sub, void *full,
+                                   void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Introspection for measurement                                       */
+/* ------------------------------------------------------------------ */
+/* Algorithmic HBM bytes of one drf_rhs_dev launch at the current mesh. */
+double drf_rhs_bytes(const drf_ctx *ctx);
+int64_t drf_num_elements(const drf_ctx *ctx);
+int drf_order(const drf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

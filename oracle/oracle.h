@@ -1,0 +1,91 @@
+/*
+ * DRIFTER DG right-hand-side ORACLE -- test infrastructure only.
+ *
+ * This is a plain-C, FP64 restatement of the reference CPU algorithm
+ * (olelindberg/drifter) for the DG right-hand-side path.  It exists to CHECK
+ * the HIP implementation in drifter_amd/; it is never linked into, called by,
+ * or used as a fallback for the product.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load liboracle.so.
+ *
+ * Every function cites the reference file:line it follows.  Layout
+ * conventions are the reference's:
+ *   - DOF index d = i + n*(j + n*k), xi fastest (include/core/types.hpp:83)
+ *   - packed state U = [elem][Hu|Hv|eta|HT|HS][ndof]
+ *     (src/solver/simulation_driver.cpp:199-213)
+ *   - full per-element state (this oracle's flat form of PrimitiveState,
+ *     include/physics/primitive_equations.hpp:82-103):
+ *     [elem][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][ndof]
+ *
+ * Pinning status of each function is listed in the header comment of the
+ * file that implements it.
+ */
+#ifndef DRIFTER_ORACLE_H
+#define DRIFTER_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_N 9 /* order <= 8 (MAX_POLYNOMIAL_ORDER, include/core/types.hpp:131) */
+
+enum { ORC_F_HU = 0, ORC_F_HV, ORC_F_ETA, ORC_F_HT, ORC_F_HS, ORC_NCONS = 5 };
+enum {
+    ORC_S_HU = 0, ORC_S_HV, ORC_S_ETA, ORC_S_HT, ORC_S_HS, ORC_S_H, ORC_S_U,
+    ORC_S_V, ORC_S_T, ORC_S_S, ORC_S_OMEGA, ORC_S_RHO, ORC_NFULL = 12
+};
+
+/* OceanConstants + DiffusivityParams (primitive_equations.hpp:41-49,106-117). */
+typedef struct {
+    double g;         /* 9.81 */
+    double nu_h;      /* 1.0 */
+    double nu_v;      /* 1e-4 */
+    double kappa_v_T; /* 1e-5 */
+    double kappa_v_S; /* 1e-5 */
+    double rho_0;     /* 1025 */
+    double alpha;     /* 2e-4 */
+    double beta;      /* 7.6e-4 */
+} orc_physics;
+
+void orc_default_physics(orc_physics *p);
+
+/* ---- A1: 1D bases (src/dg/basis_hexahedron.cpp) ---- */
+int orc_lgl_nodes(int n, double *nodes, double *weights);
+int orc_gl_nodes(int n, double *nodes, double *weights);
+void orc_barycentric_weights(int n, const double *nodes, double *bw);
+void orc_derivative_matrix(int n, const double *nodes, double *D /* n*n row-major */);
+/* Dense Kronecker D_xi, D_eta, D_zeta (ndof*ndof row-major each). */
+void orc_kron_diff(int order, const double *D1, double *Dxi, double *Deta, double *Dzeta);
+void orc_legendre(int n, double x, double *L, double *dL);
+
+/* ---- A2-A8: PrimitiveEquationsSolver::compute_rhs on the packed layout ---- */
+/* f: Coriolis parameter at every DOF (E*ndof), h: bathymetry (E*ndof). */
+int orc_rhs(int order, int64_t n_elem, const double *U, const double *h, const double *f,
+            const orc_physics *phys, double *dUdt);
+/* Same arithmetic, element range [e0, e1) only (used for bounded CPU timing). */
+int orc_rhs_range(int order, int64_t e0, int64_t e1, const double *U, const double *h,
+                  const double *f, const orc_physics *phys, double *dUdt);
+/* PrimitiveState::update_derived on the packed layout: writes H,u,v,T,S (E*ndof each). */
+void orc_update_derived(int order, int64_t n_elem, const double *U, const double *h, double *H,
+                        double *u, double *v, double *T, double *S);
+
+/* ---- A9: time steppers (src/solver/time_stepper.cpp) ---- */
+enum { ORC_EULER = 0, ORC_RK2 = 1, ORC_SSPRK3 = 2, ORC_RK4 = 3, ORC_LSRK45 = 4, ORC_DP45 = 5 };
+int orc_rk_step(int scheme, int order, int64_t n_elem, double t, double dt, double *U,
+                const double *h, const double *f, const orc_physics *phys);
+
+/* ---- A10: compute_max_wave_speed + AdaptiveTimeController::compute_dt ---- */
+double orc_max_wave_speed(int order, int64_t n_elem, const double *H, const double *u,
+                          const double *v, int64_t elem_stride);
+double orc_compute_dt(double max_wave_speed, double dx_min, double cfl, double dt_min,
+                      double dt_max);
+
+/* ---- A11-A12: ModeSplittingSolver::step on the full-state layout ---- */
+int orc_mode_split_step(int order, int64_t n_elem, double dt_3d, int n_sub, double *full,
+                        const double *h, const double *f, const orc_physics *phys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+"""ctypes wrapper of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference's FP64 algorithm used to
+check the HIP implementation.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+EULER, RK2, SSPRK3, RK4, LSRK45, DP45 = range(6)
+
+
+class OrcPhysics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("g", "nu_h", "nu_v", "kappa_v_T", "kappa_v_S", "rho_0", "alpha", "beta")]
+
+
+_lib = None
+_PD = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_default_physics": (None, [C.POINTER(OrcPhysics)]),
+        "orc_lgl_nodes": (C.c_int, [C.c_int, _PD, _PD]),
+        "orc_gl_nodes": (C.c_int, [C.c_int, _PD, _PD]),
+        "orc_derivative_matrix": (None, [C.c_int, _PD, _PD]),
+        "orc_kron_diff": (None, [C.c_int, _PD, _PD, _PD, _PD]),
+        "orc_legendre": (None, [C.c_int, C.c_double, _PD, _PD]),
+        "orc_rhs": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, _PD, C.POINTER(OrcPhysics), _PD]),
+        "orc_rhs_range": (C.c_int, [C.c_int, C.c_int64, C.c_int64, _PD, _PD, _PD,
+                                    C.POINTER(OrcPhysics), _PD]),
+        "orc_update_derived": (None, [C.c_int, C.c_int64, _PD, _PD, _PD, _PD, _PD, _PD, _PD]),
+        "orc_rk_step": (C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, _PD, _PD,
+                                  _PD, C.POINTER(OrcPhysics)]),
+        "orc_max_wave_speed": (C.c_double, [C.c_int, C.c_int64, _PD, _PD, _PD, C.c_int64]),
+        "orc_compute_dt": (C.c_double, [C.c_double] * 5),
+        "orc_mode_split_step": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_int, _PD, _PD, _PD,
+                                          C.POINTER(OrcPhysics)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PD)
+
+
+def physics(**kw):
+    p = OrcPhysics()
+    load().orc_default_physics(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def lgl(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert load().orc_lgl_nodes(n, _p(x), _p(w)) == 0
+    return x, w
+
+
+def gl(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert load().orc_gl_nodes(n, _p(x), _p(w)) == 0
+    return x, w
+
+
+def dmat(nodes):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    n = len(nodes)
+    D = np.zeros(n * n)
+    load().orc_derivative_matrix(n, _p(nodes), _p(D))
+    return D.reshape(n, n)
+
+
+def kron(order, D1):
+    n = order + 1
+    nd = n ** 3
+    D1 = np.ascontiguousarray(D1, dtype=np.float64)
+    a = np.zeros(nd * nd)
+    b = np.zeros(nd * nd)
+    c = np.zeros(nd * nd)
+    load().orc_kron_diff(order, _p(D1), _p(a), _p(b), _p(c))
+    return a.reshape(nd, nd), b.reshape(nd, nd), c.reshape(nd, nd)
+
+
+def rhs(order, U, h, f, phys=None, e0=None, e1=None):
+    n = order + 1
+    nd = n ** 3
+    E = U.size // (5 * nd)
+    out = np.zeros_like(U)
+    p = phys or physics()
+    if e0 is None:
+        rc = load().orc_rhs(order, E, _p(U), _p(h), _p(f), C.byref(p), _p(out))
+    else:
+        rc = load().orc_rhs_range(order, e0, e1, _p(U), _p(h), _p(f), C.byref(p), _p(out))
+    assert rc == 0
+    return out
+
+
+def rk_step(scheme, order, t, dt, U, h, f, phys=None):
+    n = order + 1
+    E = U.size // (5 * n ** 3)
+    V = U.copy()
+    p = phys or physics()
+    assert load().orc_rk_step(scheme, order, E, t, dt, _p(V), _p(h), _p(f), C.byref(p)) == 0
+    return V
+
+
+def derived(order, U, h):
+    n = order + 1
+    nd = n ** 3
+    E = U.size // (5 * nd)
+    arrs = [np.zeros(E * nd) for _ in range(5)]
+    load().orc_update_derived(order, E, _p(U), _p(h), *[_p(a) for a in arrs])
+    return arrs  # H, u, v, T, S
+
+
+def max_wave_speed(order, H, u, v, stride=None):
+    n = order + 1
+    nd = n ** 3
+    stride = stride or nd
+    E = H.size // stride
+    return load().orc_max_wave_speed(order, E, _p(H), _p(u), _p(v), stride)
+
+
+def compute_dt(c, dx_min, cfl, dt_min, dt_max):
+    return load().orc_compute_dt(c, dx_min, cfl, dt_min, dt_max)
+
+
+def mode_split_step(order, dt3, nsub, full, h, f, phys=None):
+    n = order + 1
+    E = full.size // (12 * n ** 3)
+    V = full.copy()
+    p = phys or physics()
+    assert load().orc_mode_split_step(order, E, dt3, nsub, _p(V), _p(h), _p(f), C.byref(p)) == 0
+    return V

@@ -1,0 +1,61 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/drifter_hip/drf.h declares, and its host-only entry points behave
+(no GPU is touched here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from drifter_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for hdr in os.listdir(os.path.join(ROOT, "include", "drifter_hip")):
+        if hdr.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", "drifter_hip", hdr)).read()
+            decl = r"^\s*(?:const\s+)?(?:drf_status|void|double|int64_t|int|char)\s*\*?\s*(drf_\w+)\s*\("
+            names |= set(re.findall(decl, txt, flags=re.M))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 20
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers the header exactly
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+
+
+def test_invalid_order_is_rejected_before_touching_a_device():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.drf_create(0, 0, 0, C.byref(h)) == _lib.DRF_ERR_UNSUPPORTED
+    assert b"order" in lib.drf_last_error()
+    assert lib.drf_create(9, 0, 0, C.byref(h)) == _lib.DRF_ERR_UNSUPPORTED
+    assert lib.drf_create(3, 7, 0, C.byref(h)) == _lib.DRF_ERR_ARG
+    assert not h.value
+
+
+def test_compute_dt_matches_oracle(orc):
+    lib = _lib.load()
+    for c in (0.0, 1e-15, 3.7, 1e3, 1e12):
+        for cfl in (0.5, 0.9):
+            a = lib.drf_compute_dt(c, 0.4, cfl, 1e-6, 3600.0)
+            b = orc.compute_dt(c, 0.4, cfl, 1e-6, 3600.0)
+            assert a == b
+
+
+def test_basis_export():
+    lib = _lib.load()
+    x = np.zeros(5)
+    w = np.zeros(5)
+    D = np.zeros(25)
+    assert lib.drf_basis(4, _lib.dptr(x), _lib.dptr(w), _lib.dptr(D)) == 0
+    assert x[0] == -1.0 and x[-1] == 1.0 and abs(w.sum() - 2) < 1e-15
+    assert lib.drf_basis(0, None, None, None) == _lib.DRF_ERR_ARG
