@@ -49,8 +49,8 @@ def time_launches(fn, steps, warmup, stream):
 def rel_l2_fields(a, b, E, nd):
     import numpy as np
 
-    A = a.reshape(5, E, nd)
-    B = b.reshape(5, E, nd)
+    A = a.reshape(5, -1)
+    B = b.reshape(5, -1)
     return [float(np.linalg.norm(A[f] - B[f]) / np.linalg.norm(B[f])) for f in range(5)]
 
 
@@ -111,9 +111,9 @@ def main():
 
     s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U = torch.from_numpy(synth.to_soa(case.U, E, nd)).to(dev)
+    U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
     R = torch.empty_like(U)
-    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_SOA, stream),
+    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
                        args.steps, args.warmup, stream)
     gdof = dof * 5 / (ms * 1e-3) / 1e9
     bytes_per = s64.rhs_bytes()
@@ -126,7 +126,7 @@ def main():
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
     R32 = torch.empty_like(U32)
-    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_SOA, stream),
+    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
                          args.steps, args.warmup, stream)
     err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
     log(f"mixed RHS: {ms32 * 1e3:.1f} us, max rel L2 vs FP64 {max(err32):.2e}")
@@ -134,13 +134,13 @@ def main():
     # RK4 steps: fused epilogue vs the reference's un-fused sequence
     dt = 1e-3
     Uw = U.clone()
-    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_SOA,
+    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
                                                     True, stream), 10, 2, stream)
     Uw = U.clone()
-    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_SOA,
+    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
                                                     False, stream), 10, 2, stream)
     Uw32 = U32.clone()
-    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_SOA,
+    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_LINES,
                                                     True, stream), 10, 2, stream)
     log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
         f"{ms_rk_m:.3f} ms")
@@ -168,7 +168,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "DG RHS, uniform 64x64x8 octree, N=4, tanh shelf, FP64, SoA in HBM",
+            "workload": "DG RHS, uniform 64x64x8 octree, N=4, tanh shelf, FP64, SoA (xi-planes) in HBM",
             "elements": E,
             "order": 4,
             "dof_per_field": dof,

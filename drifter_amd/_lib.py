@@ -18,6 +18,7 @@ DRF_ERR_RANGE = 5
 
 LAYOUT_PACKED = 0
 LAYOUT_SOA = 1
+LAYOUT_LINES = 2
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -53,6 +54,15 @@ SIGNATURES = {
     "drf_compute_dt": (_D, [_D, _D, _D, _D, _D]),
     "drf_mode_split_step": (_I, [_P, _D, _I, _PD]),
     "drf_mode_split_step_dev": (_I, [_P, _D, _I, _P, _P]),
+    "drf_mesh_create": (_I, [_D, _D, _D, _D, _D, _D, C.POINTER(_P)]),
+    "drf_mesh_destroy": (None, [_P]),
+    "drf_mesh_build_uniform": (_I, [_P, _I, _I, _I]),
+    "drf_mesh_refine": (_I, [_P, _L, _P, _P]),
+    "drf_mesh_balance": (_I, [_P]),
+    "drf_mesh_num_elements": (_L, [_P]),
+    "drf_mesh_elements": (_I, [_P, _P, _P, _P]),
+    "drf_mesh_element_bounds": (_I, [_P, _L, _PD]),
+    "drf_mesh_face_connections": (_I, [_P, _P, _P, _P, _P, _P]),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

@@ -82,16 +82,51 @@ drf_status check_ready(const drf_ctx *c)
 
 template <class T> View<T> view(void *p, int layout, const drf_ctx *c)
 {
-    View<T> v{(T *)p, 0, 0};
-    layout_strides(layout, c->E, c->nd, 5, v.fs, v.es);
+    View<T> v{(T *)p, 0, 0, 0, 0};
+    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is);
     return v;
 }
 
 template <class T> View<const T> cview(const void *p, int layout, const drf_ctx *c)
 {
-    View<const T> v{(const T *)p, 0, 0};
-    layout_strides(layout, c->E, c->nd, 5, v.fs, v.es);
+    View<const T> v{(const T *)p, 0, 0, 0, 0};
+    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is);
     return v;
+}
+
+// Per-node coefficient arrays (bathymetry, Coriolis) live on the device in
+// the LINES layout whatever layout the state uses.
+template <class T> View<const T> coef_view(const T *p, const drf_ctx *c)
+{
+    return cview<T>(p, DRF_LAYOUT_LINES, c);
+}
+
+bool valid_layout(int layout)
+{
+    return layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA || layout == DRF_LAYOUT_LINES;
+}
+
+// element-major reference order h[e*nd + i + n L] -> LINES order
+template <class T> std::vector<T> to_lines(const double *h, int64_t E, int n)
+{
+    const int64_t n2 = (int64_t)n * n, nd = n2 * n;
+    std::vector<T> out(E * nd);
+    for (int64_t e = 0; e < E; ++e)
+        for (int64_t L = 0; L < n2; ++L)
+            for (int i = 0; i < n; ++i)
+                out[i * (E * n2) + e * n2 + L] = (T)h[e * nd + L * n + i];
+    return out;
+}
+
+// The volume kernel addresses each field through a 32-bit per-lane byte
+// offset (element and line part); it must fit for every view of the call.
+bool lane_offsets_fit(const drf_ctx *c, int layout)
+{
+    int64_t fs, es, ls, is;
+    layout_strides(layout, c->E, c->n, 5, fs, es, ls, is);
+    const int64_t n2 = (int64_t)c->n * c->n;
+    const int64_t max_off = ((c->E - 1) * es + (n2 - 1) * ls + 1) * (int64_t)elem_size(c);
+    return max_off < (int64_t(1) << 32);
 }
 
 // One volume-kernel launch at the context's precision.
@@ -99,6 +134,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
                   const void *U0, void *Y, void *A, const void *X, double c0, double c1,
                   double c2, double c3, hipStream_t s)
 {
+    DRF_REQUIRE(lane_offsets_fit(c, layout), DRF_ERR_UNSUPPORTED,
+                "mesh too large for 32-bit per-field offsets in this layout (use LINES or SOA)");
     hipError_t err;
     if (c->precision == DRF_PREC_FP64) {
         Epi<double> epi{};
@@ -113,7 +150,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c1 = c1;
         epi.c2 = c2;
         epi.c3 = c3;
-        VolGeom<double> geo{c->E, c->d_h, c->d_f, c->f0};
+        VolGeom<double> geo{c->E, coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
+                            c->f0};
         err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
                                  s);
     } else {
@@ -129,7 +167,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c1 = c1;
         epi.c2 = c2;
         epi.c3 = c3;
-        VolGeom<float> geo{c->E, c->d_h32, c->d_f32, c->f0};
+        VolGeom<float> geo{c->E, coef_view<float>(c->d_h32, c), coef_view<float>(c->d_f32, c),
+                           c->f0};
         err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
                                  s);
     }
@@ -173,7 +212,7 @@ drf_status rk_unfused(drf_ctx *c, int scheme, double dt, double *U, int layout, 
             lc.c[lc.nk] = kc.second;
             ++lc.nk;
         }
-        hipError_t e = launch_lincomb(c->E, c->nd, lc, s);
+        hipError_t e = launch_lincomb(c->E, c->n, lc, s);
         return e == hipSuccess ? DRF_OK : fail(DRF_ERR_HIP, hipGetErrorString(e));
     };
 #define TRY(x)                                                                               \
@@ -291,9 +330,9 @@ drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipS
         break;
     case DRF_RK_RK4:
         TRY(volume(c, U, layout, EPI_STAGE, 1, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 0, 0, s));
-        TRY(volume(c, Ut, layout, EPI_STAGE, 2, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 2.0, 0,
-                   s));
-        TRY(volume(c, Ut, layout, EPI_STAGE, 2, nullptr, U, Ut, A, nullptr, 0, dt, 2.0, 0, s));
+        TRY(volume(c, Ut, layout, EPI_STAGE_ACC, 0, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 2.0,
+                   0, s));
+        TRY(volume(c, Ut, layout, EPI_STAGE_ACC, 0, nullptr, U, Ut, A, nullptr, 0, dt, 2.0, 0, s));
         TRY(volume(c, Ut, layout, EPI_RK4FIN, 0, nullptr, U, U, A, nullptr, 0, dt / 6.0, 0, 0,
                    s));
         break;
@@ -321,8 +360,8 @@ drf_status wave_speed(drf_ctx *c, const void *U, int layout, double *out, hipStr
     drf_status st = ensure_red(c, npart + 1);
     if (st)
         return st;
-    hipError_t e = launch_wave_speed(c->order, c->E, cview<double>(U, layout, c), c->d_h,
-                                     c->d_red + 1, npart, s);
+    hipError_t e = launch_wave_speed(c->order, c->E, cview<double>(U, layout, c),
+                                     coef_view<double>(c->d_h, c), c->d_red + 1, npart, s);
     if (e == hipSuccess)
         e = launch_max_reduce(c->d_red + 1, npart, c->d_red, s);
     if (e == hipSuccess)
@@ -454,29 +493,30 @@ drf_status drf_initialize(drf_ctx *c, int64_t E, const double *h, const double *
     c->d_h32 = c->d_f32 = nullptr;
     c->E = E;
     c->f0 = f0;
-    DRF_HIP_CHECK(hipMalloc(&c->d_h, sizeof(double) * len));
-    DRF_HIP_CHECK(hipMemcpy(c->d_h, h, sizeof(double) * len, hipMemcpyHostToDevice));
-    std::vector<double> f;
+    {
+        std::vector<double> hl = to_lines<double>(h, E, c->n);
+        DRF_HIP_CHECK(hipMalloc(&c->d_h, sizeof(double) * len));
+        DRF_HIP_CHECK(hipMemcpy(c->d_h, hl.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+        if (c->precision == DRF_PREC_MIXED) {
+            std::vector<float> h32 = to_lines<float>(h, E, c->n);
+            DRF_HIP_CHECK(hipMalloc(&c->d_h32, sizeof(float) * len));
+            DRF_HIP_CHECK(
+                hipMemcpy(c->d_h32, h32.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+        }
+    }
     if (beta != 0.0) {
         // CoriolisParameter::compute, primitive_equations.hpp:65-70
-        f.resize(len);
+        std::vector<double> f(len);
         for (int64_t i = 0; i < len; ++i)
             f[i] = f0 + beta * (y[i] - y0);
+        std::vector<double> fl = to_lines<double>(f.data(), E, c->n);
         DRF_HIP_CHECK(hipMalloc(&c->d_f, sizeof(double) * len));
-        DRF_HIP_CHECK(hipMemcpy(c->d_f, f.data(), sizeof(double) * len, hipMemcpyHostToDevice));
-    }
-    if (c->precision == DRF_PREC_MIXED) {
-        std::vector<float> tmp(len);
-        for (int64_t i = 0; i < len; ++i)
-            tmp[i] = (float)h[i];
-        DRF_HIP_CHECK(hipMalloc(&c->d_h32, sizeof(float) * len));
-        DRF_HIP_CHECK(hipMemcpy(c->d_h32, tmp.data(), sizeof(float) * len, hipMemcpyHostToDevice));
-        if (!f.empty()) {
-            for (int64_t i = 0; i < len; ++i)
-                tmp[i] = (float)f[i];
+        DRF_HIP_CHECK(hipMemcpy(c->d_f, fl.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+        if (c->precision == DRF_PREC_MIXED) {
+            std::vector<float> f32 = to_lines<float>(f.data(), E, c->n);
             DRF_HIP_CHECK(hipMalloc(&c->d_f32, sizeof(float) * len));
             DRF_HIP_CHECK(
-                hipMemcpy(c->d_f32, tmp.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+                hipMemcpy(c->d_f32, f32.data(), sizeof(float) * len, hipMemcpyHostToDevice));
         }
     }
     return DRF_OK;
@@ -489,8 +529,7 @@ drf_status drf_rhs_dev(drf_ctx *c, double t, const void *U, void *dUdt, int layo
     if (st)
         return st;
     DRF_REQUIRE(U && dUdt, DRF_ERR_ARG, "null state pointer");
-    DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA, DRF_ERR_ARG,
-                "unknown layout");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
     return rhs_into(c, U, dUdt, layout, pick_stream(c, stream));
 }
 
@@ -526,8 +565,7 @@ drf_status drf_rk_step_dev(drf_ctx *c, int scheme, double t, double dt, void *U,
     if (st)
         return st;
     DRF_REQUIRE(U, DRF_ERR_ARG, "null state pointer");
-    DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA, DRF_ERR_ARG,
-                "unknown layout");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
     DRF_REQUIRE(scheme >= DRF_RK_EULER && scheme <= DRF_RK_DP45, DRF_ERR_ARG,
                 "unknown Runge-Kutta scheme " + std::to_string(scheme));
     hipStream_t s = pick_stream(c, stream);
@@ -572,6 +610,7 @@ drf_status drf_max_wave_speed_dev(drf_ctx *c, const void *U, int layout, double 
     if (st)
         return st;
     DRF_REQUIRE(U && c_max, DRF_ERR_ARG, "null argument");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
     return wave_speed(c, U, layout, c_max, pick_stream(c, stream));
 }
 
@@ -630,7 +669,8 @@ drf_status drf_mode_split_step_dev(drf_ctx *c, double dt3, int nsub, void *full,
         return st;
     DRF_REQUIRE(full, DRF_ERR_ARG, "null state pointer");
     DRF_REQUIRE(nsub >= 1, DRF_ERR_ARG, "subcycles must be >= 1");
-    hipError_t e = launch_mode_split(c->order, c->E, dt3, nsub, (double *)full, c->d_h, c->d_f,
+    hipError_t e = launch_mode_split(c->order, c->E, dt3, nsub, (double *)full,
+                                     coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
                                      c->f0, c->ops, c->phys, pick_stream(c, stream));
     if (e != hipSuccess)
         return fail(DRF_ERR_HIP, std::string("mode split launch: ") + hipGetErrorString(e));

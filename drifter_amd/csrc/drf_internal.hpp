@@ -29,10 +29,8 @@ void derivative_matrix(int n, const double *x, double *D);
 
 // Operator constants passed to every kernel by value (they land in SGPRs).
 template <class T> struct OpsT {
-    T D[kMaxN * kMaxN];      // 1D derivative matrix, row-major
-    T sw[kMaxN];             // 0.5 * LGL weights (sigma weights)
-    double D64[kMaxN * kMaxN]; // FP64 copies: column integrals of the free-surface
-    double sw64[kMaxN];        // equation stay in FP64 at every precision
+    T D[kMaxN * kMaxN]; // 1D derivative matrix, row-major (n x n used)
+    T sw[kMaxN];        // 0.5 * LGL weights (sigma weights)
 };
 using Ops = OpsT<double>;
 
@@ -40,32 +38,54 @@ struct Phys {
     double g, nu_h, nu_v, kT, kS, rho0, alpha, beta;
 };
 
-// Strided view of a 5-field state: element e, field f, node d lives at
-// base[f * fs + e * es + d].
+// Strided view of a multi-field nodal state.  Node d = i + n L of element e
+// (i the xi index, L = j + n k the grid line) in field f lives at
+// p[f * fs + e * es + L * ls + i * is].
 template <class T> struct View {
     T *p;
-    int64_t fs, es;
+    int64_t fs, es, ls, is;
 };
 
-inline void layout_strides(int layout, int64_t E, int nd, int nfields, int64_t &fs,
-                           int64_t &es)
+// Strides of the public layouts (drf.h).  LINES is the kernels' native
+// layout: for a fixed xi index, consecutive (element, line) pairs are
+// consecutive in memory, so a wavefront whose lanes own consecutive lines
+// touches one contiguous span per load or store.
+inline void layout_strides(int layout, int64_t E, int n, int nfields, int64_t &fs, int64_t &es,
+                           int64_t &ls, int64_t &is)
 {
-    if (layout == DRF_LAYOUT_SOA) {
+    const int64_t nd = (int64_t)n * n * n, n2 = (int64_t)n * n;
+    if (layout == DRF_LAYOUT_LINES) {
+        fs = E * nd;
+        es = n2;
+        ls = 1;
+        is = E * n2;
+    } else if (layout == DRF_LAYOUT_SOA) {
         fs = E * nd;
         es = nd;
+        ls = n;
+        is = 1;
     } else {
         fs = nd;
         es = (int64_t)nfields * nd;
+        ls = n;
+        is = 1;
     }
+}
+
+template <class T> __host__ __device__ inline int64_t node_offset(const View<T> &v, int64_t e, int L,
+                                                                  int i)
+{
+    return e * v.es + (int64_t)L * v.ls + (int64_t)i * v.is;
 }
 
 // Epilogue of the volume kernel: what is done with the tendency k of each node.
 enum EpiMode : int {
-    EPI_RHS = 0,     // R = k
-    EPI_STAGE = 1,   // Y = U0 + c1 k ; A: acc_mode 0 none, 1 A = k, 2 A = A + c2 k
-    EPI_RK4FIN = 2,  // Y = U0 + c1 (A + k)
-    EPI_COMB = 3,    // Y = c0 U0 + c1 (X + c2 k), X = the RHS input state
-    EPI_LSRK = 4,    // A = c2 A + c1 k (acc_mode 1: A = c1 k) ; Y = X + c3 A
+    EPI_RHS = 0,       // R = k
+    EPI_STAGE = 1,     // Y = U0 + c1 k ; acc_mode 1 also writes A = k
+    EPI_STAGE_ACC = 2, // A = A + c2 k ; Y = U0 + c1 k
+    EPI_RK4FIN = 3,    // Y = U0 + c1 (A + k)
+    EPI_COMB = 4,      // Y = c0 U0 + c1 (X + c2 k), X = the RHS input state
+    EPI_LSRK = 5,      // A = c2 A + c1 k (acc_mode 1: A = c1 k) ; Y = X + c3 A
 };
 
 template <class T> struct Epi {
@@ -81,9 +101,9 @@ template <class T> struct Epi {
 
 template <class T> struct VolGeom {
     int64_t E;
-    const T *h; // E*nd bathymetry (element-major)
-    const T *f; // E*nd Coriolis or nullptr
-    double f0;  // used when f == nullptr
+    View<const T> h; // bathymetry at every node (fs unused)
+    View<const T> f; // Coriolis parameter at every node; f.p == nullptr -> f0
+    double f0;
 };
 
 // Launchers (volume.hip).  Return hipError_t of the launch.
@@ -96,9 +116,7 @@ hipError_t launch_volume_fp32(int order, const VolGeom<float> &geo, const Ops &o
                               hipStream_t s);
 
 // Vector kernels (stepper.hip)
-hipError_t launch_axpy_fields(int64_t E, int nd, View<double> Y, View<const double> X, double a,
-                              View<const double> K, hipStream_t s); // Y = X + a K
-hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, const double *h,
+hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, View<const double> h,
                              double *partial, int64_t npart, hipStream_t s);
 hipError_t launch_wave_speed_derived(int order, int64_t E, const double *H, const double *u,
                                      const double *v, int64_t es, double *partial,
@@ -107,8 +125,8 @@ hipError_t launch_max_reduce(const double *partial, int64_t npart, double *out, 
 
 // Mode split (modesplit.hip)
 hipError_t launch_mode_split(int order, int64_t E, double dt3, int nsub, double *full,
-                             const double *h, const double *f, double f0, const Ops &ops,
-                             const Phys &ph, hipStream_t s);
+                             View<const double> h, View<const double> f, double f0,
+                             const Ops &ops, const Phys &ph, hipStream_t s);
 
 } // namespace drf
 
@@ -121,9 +139,10 @@ struct drf_ctx {
     drf::Ops ops{};
     drf::Phys phys{};
     int64_t E = 0;
-    double *d_h = nullptr;      // E*nd
-    double *d_f = nullptr;      // E*nd or null when uniform
-    float *d_h32 = nullptr;     // mixed precision copy
+    // bathymetry / Coriolis at every node, stored in the LINES layout
+    double *d_h = nullptr;
+    double *d_f = nullptr;      // null when f is uniform (beta == 0)
+    float *d_h32 = nullptr;     // mixed precision copies
     float *d_f32 = nullptr;
     double f0 = 0.0;
     // host-packed staging and RK work space (grown on demand)

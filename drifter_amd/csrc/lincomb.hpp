@@ -13,6 +13,6 @@ struct LinComb {
     double alpha, beta, gamma;
 };
 
-hipError_t launch_lincomb(int64_t E, int nd, const LinComb &lc, hipStream_t s);
+hipError_t launch_lincomb(int64_t E, int n, const LinComb &lc, hipStream_t s);
 
 } // namespace drf

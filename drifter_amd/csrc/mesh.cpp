@@ -1,0 +1,381 @@
+// Directional octree mesh, 2:1 balance and face connectivity (host side).
+//
+// Replaces OctreeAdapter (include/mesh/octree_adapter.hpp:103-273):
+// build_uniform (src/mesh/octree_adapter.cpp:23-84), refine (:184-203),
+// balance (:138-182), element_bounds (:234-239) and build_face_connections
+// (:369-389 via get_neighbor :259-359).  Element numbering, bounds arithmetic,
+// tie-breaks and connection types are the reference's, so the flat index maps
+// handed to the face kernels are bit-identical to what the reference would
+// build:
+//   - children are created iz -> iy -> ix with bounds min + i * (max - min)/c;
+//   - elements are numbered in depth-first leaf order;
+//   - a face's neighbour is the leaf containing the centre of the would-be
+//     same-size neighbour (inclusive bounds, first child wins on ties),
+//     reported as SameLevel; only when that descent stops at an interior node
+//     are the fine leaves sharing the face listed in breadth-first order;
+//   - a face is a boundary face when it lies within 1e-12 of the domain.
+// Storage differs from the reference's pointer tree: nodes live in one
+// vector with contiguous children, and traversals use explicit stacks/queues.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "drf_internal.hpp"
+
+struct drf_mesh {
+    struct Node {
+        double b[6];      // xmin xmax ymin ymax zmin zmax
+        int lev[3];       // per-axis refinement level
+        uint64_t morton;  // directional Morton code
+        int mask;         // refinement mask used for the children (bit0 X, bit1 Y, bit2 Z)
+        int64_t child0;   // first child (children are contiguous), -1 for a leaf
+        int nchild;
+        int64_t leaf;     // element number, -1 if not a leaf
+    };
+    double dom[6];
+    std::vector<Node> nodes;
+    std::vector<int64_t> leaves;
+};
+
+namespace {
+
+using Node = drf_mesh::Node;
+
+constexpr uint64_t kXMask = 0x1249249249249249ULL; // Morton3D masks, include/mesh/morton.hpp:33-35
+constexpr uint64_t kYMask = 0x2492492492492492ULL;
+constexpr uint64_t kZMask = 0x4924924924924924ULL;
+
+// MortonUtil::child_{x,y,z} / refine, include/mesh/morton.hpp:104-143
+uint64_t morton_child(uint64_t code, uint64_t mask, uint64_t bitval, bool set)
+{
+    const uint64_t bits = ((code & mask) << 3) | (set ? bitval : 0ULL);
+    return (code & ~mask) | (bits & mask);
+}
+
+void create_children(drf_mesh &m, int64_t id)
+{
+    if (m.nodes[id].nchild)
+        return;
+    const int mask = m.nodes[id].mask;
+    const bool rx = mask & 1, ry = mask & 2, rz = mask & 4;
+    const int cx = rx ? 2 : 1, cy = ry ? 2 : 1, cz = rz ? 2 : 1;
+    const Node p = m.nodes[id];
+    const double dx = (p.b[1] - p.b[0]) / cx, dy = (p.b[3] - p.b[2]) / cy,
+                 dz = (p.b[5] - p.b[4]) / cz;
+    const int64_t first = (int64_t)m.nodes.size();
+    for (int iz = 0; iz < cz; ++iz)
+        for (int iy = 0; iy < cy; ++iy)
+            for (int ix = 0; ix < cx; ++ix) {
+                Node c{};
+                uint64_t code = p.morton;
+                if (rx)
+                    code = morton_child(code, kXMask, 0x1, ix);
+                if (ry)
+                    code = morton_child(code, kYMask, 0x2, iy);
+                if (rz)
+                    code = morton_child(code, kZMask, 0x4, iz);
+                c.morton = code;
+                c.lev[0] = p.lev[0] + (rx ? 1 : 0);
+                c.lev[1] = p.lev[1] + (ry ? 1 : 0);
+                c.lev[2] = p.lev[2] + (rz ? 1 : 0);
+                c.b[0] = p.b[0] + ix * dx;
+                c.b[1] = p.b[0] + (ix + 1) * dx;
+                c.b[2] = p.b[2] + iy * dy;
+                c.b[3] = p.b[2] + (iy + 1) * dy;
+                c.b[4] = p.b[4] + iz * dz;
+                c.b[5] = p.b[4] + (iz + 1) * dz;
+                c.child0 = -1;
+                c.leaf = -1;
+                m.nodes.push_back(c);
+            }
+    m.nodes[id].child0 = first;
+    m.nodes[id].nchild = cx * cy * cz;
+}
+
+void number_leaves(drf_mesh &m)
+{
+    for (auto &nd : m.nodes)
+        nd.leaf = -1;
+    m.leaves.clear();
+    std::vector<int64_t> stack{0};
+    while (!stack.empty()) {
+        const int64_t id = stack.back();
+        stack.pop_back();
+        const Node &nd = m.nodes[id];
+        if (!nd.nchild) {
+            m.nodes[id].leaf = (int64_t)m.leaves.size();
+            m.leaves.push_back(id);
+            continue;
+        }
+        for (int c = nd.nchild - 1; c >= 0; --c) // depth first, children in creation order
+            stack.push_back(nd.child0 + c);
+    }
+}
+
+bool inside(const double *b, const double *p)
+{
+    return p[0] >= b[0] && p[0] <= b[1] && p[1] >= b[2] && p[1] <= b[3] && p[2] >= b[4] &&
+           p[2] <= b[5];
+}
+
+// centre of the same-size box across face f (find_neighbor_same_or_coarser :530-558)
+void across(const double *b, int f, double *p)
+{
+    p[0] = 0.5 * (b[0] + b[1]);
+    p[1] = 0.5 * (b[2] + b[3]);
+    p[2] = 0.5 * (b[4] + b[5]);
+    const int a = f / 2;
+    const double lo = b[2 * a], hi = b[2 * a + 1];
+    p[a] = (f % 2) ? hi + 0.5 * (hi - lo) : lo - 0.5 * (hi - lo);
+}
+
+// deepest node containing the point, or -1 outside the domain
+int64_t locate(const drf_mesh &m, const double *p)
+{
+    if (!inside(m.dom, p))
+        return -1;
+    int64_t cur = 0;
+    while (m.nodes[cur].nchild) {
+        const Node &nd = m.nodes[cur];
+        int64_t next = -1;
+        for (int c = 0; c < nd.nchild; ++c)
+            if (inside(m.nodes[nd.child0 + c].b, p)) {
+                next = nd.child0 + c;
+                break;
+            }
+        if (next < 0)
+            break;
+        cur = next;
+    }
+    return cur;
+}
+
+int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
+{
+    double p[3];
+    across(m.nodes[id].b, f, p);
+    return locate(m, p);
+}
+
+void balance(drf_mesh &m)
+{
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        const std::vector<int64_t> snapshot = m.leaves;
+        for (int64_t id : snapshot)
+            for (int f = 0; f < 6; ++f) {
+                const int64_t nb = neighbour_node(m, id, f);
+                if (nb < 0)
+                    continue;
+                int need = 0;
+                for (int a = 0; a < 3; ++a)
+                    if (m.nodes[id].lev[a] - m.nodes[nb].lev[a] > 1)
+                        need |= 1 << a;
+                if (need && !m.nodes[nb].nchild) {
+                    m.nodes[nb].mask = need;
+                    create_children(m, nb);
+                    changed = true;
+                }
+            }
+        if (changed)
+            number_leaves(m);
+    }
+}
+
+int conn_type(const int *coarse, const int *fine, int f)
+{
+    // get_connection_type, include/dg/face_connection.hpp:110-152
+    const int t1 = (f <= 1) ? 1 : 0, t2 = (f <= 3) ? 2 : 1;
+    const int d1 = fine[t1] - coarse[t1], d2 = fine[t2] - coarse[t2];
+    if (d1 == 0 && d2 == 0)
+        return 0; // SameLevel
+    if (d1 == 1 && d2 == 0)
+        return 1; // Fine2x1
+    if (d1 == 0 && d2 == 1)
+        return 2; // Fine1x2
+    return 5;     // Fine2x2 (also the 1:3 fallback)
+}
+
+} // namespace
+
+extern "C" {
+
+drf_status drf_mesh_create(double xmin, double xmax, double ymin, double ymax, double zmin,
+                           double zmax, drf_mesh **out)
+{
+    if (!out)
+        return drf::fail(DRF_ERR_ARG, "null output pointer");
+    if (!(xmax > xmin && ymax > ymin && zmax > zmin))
+        return drf::fail(DRF_ERR_ARG, "empty domain");
+    drf_mesh *m = new drf_mesh();
+    const double d[6] = {xmin, xmax, ymin, ymax, zmin, zmax};
+    std::memcpy(m->dom, d, sizeof(d));
+    Node root{};
+    std::memcpy(root.b, d, sizeof(d));
+    root.child0 = -1;
+    root.mask = 7;
+    m->nodes.push_back(root);
+    number_leaves(*m);
+    *out = m;
+    return DRF_OK;
+}
+
+void drf_mesh_destroy(drf_mesh *m) { delete m; }
+
+drf_status drf_mesh_build_uniform(drf_mesh *m, int nx, int ny, int nz)
+{
+    if (!m || nx < 1 || ny < 1 || nz < 1)
+        return drf::fail(DRF_ERR_ARG, "invalid mesh or element counts");
+    int lv[3] = {0, 0, 0};
+    const int cnt[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a)
+        for (int c = 1; c < cnt[a]; c *= 2)
+            ++lv[a];
+    Node root = m->nodes[0];
+    root.child0 = -1;
+    root.nchild = 0;
+    root.mask = 7;
+    root.lev[0] = root.lev[1] = root.lev[2] = 0;
+    root.morton = 0;
+    m->nodes.assign(1, root);
+    // breadth-first construction (:49-81)
+    std::vector<int64_t> queue{0};
+    for (size_t qh = 0; qh < queue.size(); ++qh) {
+        const int64_t id = queue[qh];
+        int mask = 0;
+        for (int a = 0; a < 3; ++a)
+            if (m->nodes[id].lev[a] < lv[a])
+                mask |= 1 << a;
+        if (!mask)
+            continue;
+        m->nodes[id].mask = mask;
+        create_children(*m, id);
+        for (int c = 0; c < m->nodes[id].nchild; ++c)
+            queue.push_back(m->nodes[id].child0 + c);
+    }
+    number_leaves(*m);
+    return DRF_OK;
+}
+
+drf_status drf_mesh_refine(drf_mesh *m, int64_t count, const int64_t *elems, const uint8_t *masks)
+{
+    if (!m || (count > 0 && (!elems || !masks)))
+        return drf::fail(DRF_ERR_ARG, "null argument");
+    for (int64_t i = 0; i < count; ++i) {
+        const int64_t e = elems[i];
+        if (e < 0 || e >= (int64_t)m->leaves.size())
+            continue; // the reference skips bad indices (:190-191)
+        const int64_t id = m->leaves[e];
+        if (m->nodes[id].nchild)
+            continue;
+        m->nodes[id].mask = masks[i] & 7;
+        create_children(*m, id);
+    }
+    number_leaves(*m);
+    balance(*m);
+    return DRF_OK;
+}
+
+drf_status drf_mesh_balance(drf_mesh *m)
+{
+    if (!m)
+        return drf::fail(DRF_ERR_ARG, "null mesh");
+    balance(*m);
+    return DRF_OK;
+}
+
+int64_t drf_mesh_num_elements(const drf_mesh *m) { return m ? (int64_t)m->leaves.size() : 0; }
+
+drf_status drf_mesh_elements(const drf_mesh *m, double *bounds, int32_t *levels, uint64_t *morton)
+{
+    if (!m)
+        return drf::fail(DRF_ERR_ARG, "null mesh");
+    for (size_t i = 0; i < m->leaves.size(); ++i) {
+        const Node &nd = m->nodes[m->leaves[i]];
+        if (bounds)
+            std::memcpy(bounds + 6 * i, nd.b, sizeof(nd.b));
+        if (levels)
+            for (int a = 0; a < 3; ++a)
+                levels[3 * i + a] = nd.lev[a];
+        if (morton)
+            morton[i] = nd.morton;
+    }
+    return DRF_OK;
+}
+
+drf_status drf_mesh_element_bounds(const drf_mesh *m, int64_t elem, double *b6)
+{
+    if (!m || !b6)
+        return drf::fail(DRF_ERR_ARG, "null argument");
+    if (elem < 0 || elem >= (int64_t)m->leaves.size())
+        return drf::fail(DRF_ERR_RANGE, "Invalid element index"); // std::out_of_range, :235-237
+    std::memcpy(b6, m->nodes[m->leaves[elem]].b, 6 * sizeof(double));
+    return DRF_OK;
+}
+
+drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *count,
+                                     int64_t *elems, int8_t *faces, int8_t *subs)
+{
+    if (!m || !type || !count || !elems || !faces || !subs)
+        return drf::fail(DRF_ERR_ARG, "null argument");
+    const int64_t E = (int64_t)m->leaves.size();
+    for (int64_t e = 0; e < E; ++e) {
+        const int64_t id = m->leaves[e];
+        const Node &nd = m->nodes[id];
+        for (int f = 0; f < 6; ++f) {
+            const int64_t k = 6 * e + f;
+            type[k] = 6; // Boundary
+            count[k] = 0;
+            for (int s = 0; s < 4; ++s) {
+                elems[4 * k + s] = -1;
+                faces[4 * k + s] = -1;
+                subs[4 * k + s] = -1;
+            }
+            const int a = f / 2;
+            const bool on_boundary = (f % 2) ? nd.b[2 * a + 1] >= m->dom[2 * a + 1] - 1e-12
+                                             : nd.b[2 * a] <= m->dom[2 * a] + 1e-12;
+            if (on_boundary)
+                continue;
+            const int64_t nb = neighbour_node(*m, id, f);
+            if (nb < 0)
+                continue;
+            const int8_t opp = (int8_t)((f % 2 == 0) ? f + 1 : f - 1);
+            if (!m->nodes[nb].nchild) {
+                type[k] = 0; // SameLevel, also when the neighbour is coarser (:331-338)
+                count[k] = 1;
+                elems[4 * k] = m->nodes[nb].leaf;
+                faces[4 * k] = opp;
+                subs[4 * k] = 0;
+                continue;
+            }
+            // fine leaves sharing the face, breadth-first from the node (:582-639)
+            std::vector<int64_t> q{nb}, fine;
+            const double plane = (f % 2) ? nd.b[2 * a + 1] : nd.b[2 * a];
+            for (size_t qh = 0; qh < q.size(); ++qh) {
+                const Node &x = m->nodes[q[qh]];
+                if (!x.nchild) {
+                    const double side = (f % 2) ? x.b[2 * a] : x.b[2 * a + 1];
+                    if (std::fabs(side - plane) < 1e-12)
+                        fine.push_back(q[qh]);
+                } else {
+                    for (int c = 0; c < x.nchild; ++c)
+                        q.push_back(x.child0 + c);
+                }
+            }
+            if (fine.empty())
+                continue;
+            type[k] = (uint8_t)conn_type(nd.lev, m->nodes[fine[0]].lev, f);
+            count[k] = (int8_t)std::min<size_t>(fine.size(), 4);
+            for (size_t s = 0; s < fine.size() && s < 4; ++s) {
+                elems[4 * k + s] = m->nodes[fine[s]].leaf;
+                faces[4 * k + s] = opp;
+                subs[4 * k + s] = (int8_t)s;
+            }
+        }
+    }
+    return DRF_OK;
+}
+
+} // extern "C"

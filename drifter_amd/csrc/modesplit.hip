@@ -31,7 +31,7 @@ __device__ __forceinline__ double hinv(double H) { return (H > 1e-10) ? 1.0 / H 
 template <int N>
 __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
     mode_split_kernel(int64_t E, double dt3, int nsub, double *__restrict__ full,
-                      const double *__restrict__ hb, const double *__restrict__ fb, double f0,
+                      const View<const double> hb, const View<const double> fb, double f0,
                       const Ops ops, const Phys ph)
 {
     using C = MsCfg<N>;
@@ -90,9 +90,10 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
         }
         eb = eta[ih]; // .head(n_h), :378
         Hb = H[ih];   // .head(n_h), :384
-        hh = hb[e * nd + ih];
-        if (fb)
-            fh = fb[e * nd + ih];
+        // bathymetry and Coriolis at DOF index ih (= node i + n L)
+        hh = hb.p[node_offset(hb, e, ih / n, ih % n)];
+        if (fb.p)
+            fh = fb.p[node_offset(fb, e, ih / n, ih % n)];
     }
 
     // ---------------- subcycle :319-332 ----------------
@@ -217,8 +218,9 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
 }
 
 template <int N>
-hipError_t ms_launch(int64_t E, double dt3, int nsub, double *full, const double *h,
-                     const double *f, double f0, const Ops &ops, const Phys &ph, hipStream_t s)
+hipError_t ms_launch(int64_t E, double dt3, int nsub, double *full, View<const double> h,
+                     View<const double> f, double f0, const Ops &ops, const Phys &ph,
+                     hipStream_t s)
 {
     using C = MsCfg<N>;
     if (E <= 0)
@@ -232,8 +234,8 @@ hipError_t ms_launch(int64_t E, double dt3, int nsub, double *full, const double
 } // namespace
 
 hipError_t launch_mode_split(int order, int64_t E, double dt3, int nsub, double *full,
-                             const double *h, const double *f, double f0, const Ops &ops,
-                             const Phys &ph, hipStream_t s)
+                             View<const double> h, View<const double> f, double f0,
+                             const Ops &ops, const Phys &ph, hipStream_t s)
 {
     switch (order) {
     case 1: return ms_launch<1>(E, dt3, nsub, full, h, f, f0, ops, ph, s);

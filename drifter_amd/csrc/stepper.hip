@@ -1,29 +1,9 @@
-// Streaming kernels around the right-hand side: the un-fused Runge-Kutta
-// vector update (the reference's own sequence, src/solver/time_stepper.cpp:12-201,
-// kept as the "plain" baseline) and the CFL wave-speed reduction
-// (SimulationDriver::compute_max_wave_speed, src/solver/simulation_driver.cpp:153-168).
+// CFL wave-speed reduction (SimulationDriver::compute_max_wave_speed,
+// src/solver/simulation_driver.cpp:153-168).
 #include "drf_internal.hpp"
 
 namespace drf {
 namespace {
-
-// Y = X + a K over the 5 state fields; one thread per node, grid-stride.
-__global__ void __launch_bounds__(256)
-    axpy_fields_kernel(int64_t E, int nd, View<double> Y, View<const double> X, double a,
-                       View<const double> K)
-{
-    const int64_t total = E * nd;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e = t / nd, d = t - e * nd;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) {
-            const double x = X.p[f * X.fs + e * X.es + d];
-            const double k = K.p[f * K.fs + e * K.es + d];
-            Y.p[f * Y.fs + e * Y.es + d] = x + a * k;
-        }
-    }
-}
 
 __device__ __forceinline__ double wave_max(double a, double b) { return a > b ? a : b; }
 
@@ -32,19 +12,23 @@ __device__ __forceinline__ double wave_max(double a, double b) { return a > b ? 
 // per element; partial[block] = max over the block's elements.
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
-    wave_speed_kernel(int64_t E, int nd, View<const double> U, const double *h, double *partial)
+    wave_speed_kernel(int64_t E, int n, View<const double> U, View<const double> h,
+                      double *partial)
 {
     __shared__ double red[WAVES];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nd = n * n * n;
     double cmax = 0.0;
     for (int64_t e = (int64_t)blockIdx.x * WAVES + w; e < E; e += (int64_t)gridDim.x * WAVES) {
         double hm = -INFINITY, um = 0.0, vm = 0.0;
         for (int d = lane; d < nd; d += 64) {
-            const double eta = U.p[2 * U.fs + e * U.es + d];
-            const double H = eta + h[e * nd + d];
+            const int L = d / n, i = d - L * n;
+            const int64_t o = node_offset(U, e, L, i);
+            const double eta = U.p[2 * U.fs + o];
+            const double H = eta + h.p[node_offset(h, e, L, i)];
             const double hi = (H > 1e-10) ? 1.0 / H : 0.0;
-            const double u = U.p[0 * U.fs + e * U.es + d] * hi;
-            const double v = U.p[1 * U.fs + e * U.es + d] * hi;
+            const double u = U.p[0 * U.fs + o] * hi;
+            const double v = U.p[1 * U.fs + o] * hi;
             hm = wave_max(hm, H);
             um = wave_max(um, fabs(u));
             vm = wave_max(vm, fabs(v));
@@ -118,25 +102,11 @@ __global__ void __launch_bounds__(256) max_reduce_kernel(const double *p, int64_
 
 } // namespace
 
-hipError_t launch_axpy_fields(int64_t E, int nd, View<double> Y, View<const double> X, double a,
-                              View<const double> K, hipStream_t s)
-{
-    const int64_t total = E * nd;
-    if (total <= 0)
-        return hipSuccess;
-    int64_t blocks = (total + 255) / 256;
-    if (blocks > 65536)
-        blocks = 65536;
-    hipLaunchKernelGGL(axpy_fields_kernel, dim3((unsigned)blocks), dim3(256), 0, s, E, nd, Y, X,
-                       a, K);
-    return hipGetLastError();
-}
-
-hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, const double *h,
+hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, View<const double> h,
                              double *partial, int64_t npart, hipStream_t s)
 {
-    const int n = order + 1, nd = n * n * n;
-    hipLaunchKernelGGL(wave_speed_kernel<4>, dim3((unsigned)npart), dim3(256), 0, s, E, nd, U, h,
+    const int n = order + 1;
+    hipLaunchKernelGGL(wave_speed_kernel<4>, dim3((unsigned)npart), dim3(256), 0, s, E, n, U, h,
                        partial);
     return hipGetLastError();
 }

@@ -1,0 +1,578 @@
+// Element-local volume right-hand side of the hydrostatic primitive
+// equations on gfx950 (MI355X), sum-factorised.
+//
+// Replaces PrimitiveEquationsSolver::compute_rhs
+// (src/physics/primitive_equations.cpp:428-442) and everything below it:
+// update_derived (:12-29), compute_momentum_rhs (:154-199), compute_tracer_rhs
+// (:201-227), compute_eta_rhs (:229-273), with the reference-coordinate
+// derivatives of DG3DElementOperator::{gradient,divergence}_reference
+// (src/dg/operators_3d.cpp:58-79) applied by sum factorisation instead of
+// dense Kronecker mat-vecs (D_xi = D (x) I (x) I, src/dg/basis_hexahedron.cpp:321-357).
+// Second derivatives D(D u) are applied as (D D) u with D^2 formed once on the
+// host: 18 one-dimensional contractions per node instead of 26.
+//
+// Work decomposition.  A workgroup owns EPB elements per batch; every element
+// is handled by n^2 threads, each owning one grid line of n nodes:
+//   phase 1  thread (j,k) owns the xi-line   -> derived state, all
+//            xi-derivatives in registers, the xi-column integrals of the eta
+//            equation, and stages what the other directions need in LDS;
+//   phase 2  thread (i,k) owns the eta-line  -> eta-derivatives from LDS,
+//            written in place over the consumed fluxes;
+//   phase 3  thread (i,j) owns the zeta-line -> zeta second derivatives,
+//            added in place;
+//   phase 4  thread (j,k) again              -> adds the staged contributions
+//            to its register partials and writes the tendency (or a fused
+//            Runge-Kutta stage update) to HBM.
+// Each value crosses LDS once per direction instead of once per derivative
+// tap.  Workgroups are persistent: while a batch is computed, the state lines
+// of the workgroup's next batch are already being loaded into registers, so
+// HBM traffic overlaps the FP64 arithmetic (occupancy is bounded by LDS).
+//
+// Index quirk reproduced on purpose: the free-surface tendency integrates
+// Hu, Hv over the xi index (d % n) and differentiates the column integrals
+// along eta and zeta, replicating the result along xi
+// (primitive_equations.cpp:244-272).  omega is identically 0 in the RHS
+// (src/solver/simulation_driver.cpp:222 -> primitive_equations.cpp:47-48), so
+// zeta fluxes vanish.
+#pragma once
+#include <type_traits>
+
+#include "drf_internal.hpp"
+
+namespace drf {
+namespace vol {
+
+// Staged LDS fields per element node.
+enum : int {
+    S_FUV = 0, // Hu*v  -> eta/zeta contribution to dHu
+    S_FVV,     // Hv*v  -> ... dHv
+    S_FTV,     // v*HT  -> ... dHT
+    S_FSV,     // v*HS  -> ... dHS
+    S_ETA,
+    S_U,
+    S_V,
+    S_H,
+    S_T,
+    S_S,
+    S_NF
+};
+
+template <int N> struct Cfg {
+    static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+    static constexpr int lds_per_elem_fp64 = (S_NF * n3 + 2 * n2) * 8;
+    static constexpr int epb_thr = (128 / n2) > 0 ? 128 / n2 : 1;
+    static constexpr int epb_lds = (65536 / lds_per_elem_fp64) > 0 ? 65536 / lds_per_elem_fp64 : 1;
+    static constexpr int EPB = epb_thr < epb_lds ? epb_thr : epb_lds;
+    static constexpr int NT = EPB * n2;
+    static constexpr int BLOCK = ((NT + 63) / 64) * 64;
+};
+
+// Operators sized for one order, passed by value (kernel-argument SGPRs).
+template <int n, class TC> struct KOps {
+    TC D[n * n];   // 1D derivative matrix, row-major
+    TC D2[n * n];  // D * D (second derivative)
+    double Dc[n * n]; // FP64 D for the free-surface column integrals
+    double sw[n];     // 0.5 * LGL weights
+};
+
+template <class TS, class TC> __device__ __forceinline__ TC ld(const TS *p) { return (TC)(*p); }
+
+// Addresses are split into a wave-uniform base (pointer + field and xi
+// offsets, all from kernel arguments: SGPRs) and a 32-bit per-lane byte
+// offset (element and line), so every access is one `saddr` load or store
+// with a single VGPR offset.  Offsets within one field must stay below 4 GiB
+// (checked on the host).
+template <class TS, class TC> __device__ __forceinline__ TC ldo(const TS *ubase, uint32_t off)
+{
+    return (TC)(*(const TS *)((const char *)ubase + off));
+}
+template <class TS> __device__ __forceinline__ void sto(TS *ubase, uint32_t off, TS v)
+{
+    *(TS *)((char *)ubase + off) = v;
+}
+template <class T> __device__ __forceinline__ uint32_t lane_off(const View<T> &v, int64_t e, int L)
+{
+    return (uint32_t)((e * v.es + (int64_t)L * v.ls) * (int64_t)sizeof(T));
+}
+
+template <class TC> __device__ __forceinline__ TC hinv_of(TC H)
+{
+    // PrimitiveState::update_derived, primitive_equations.cpp:23
+    return (H > (TC)1e-10) ? (TC)1 / H : (TC)0;
+}
+
+// y = M x along one line (n taps), M row-major.
+template <int n, class TC>
+__device__ __forceinline__ void dline(const TC *__restrict__ M, const TC (&x)[n], TC (&y)[n])
+{
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+        TC s = M[i * n] * x[0];
+#pragma unroll
+        for (int m = 1; m < n; ++m)
+            s += M[i * n + m] * x[m];
+        y[i] = s;
+    }
+}
+
+// The state lines a thread needs for phase 1.
+template <int n, class TC> struct Lines {
+    TC hu[n], hv[n], et[n], ht[n], hs[n], hb[n];
+};
+
+template <int n, class TS, class TC>
+__device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS> &U,
+                                           const View<const TS> &h, int64_t e, int line)
+{
+    const uint32_t uo = lane_off(U, e, line), ho = lane_off(h, e, line);
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+        L.hu[i] = ldo<TS, TC>(U.p + 0 * U.fs + i * U.is, uo);
+        L.hv[i] = ldo<TS, TC>(U.p + 1 * U.fs + i * U.is, uo);
+        L.et[i] = ldo<TS, TC>(U.p + 2 * U.fs + i * U.is, uo);
+        L.ht[i] = ldo<TS, TC>(U.p + 3 * U.fs + i * U.is, uo);
+        L.hs[i] = ldo<TS, TC>(U.p + 4 * U.fs + i * U.is, uo);
+        L.hb[i] = ldo<TS, TC>(h.p + i * h.is, ho);
+    }
+}
+
+template <int N, class TS, class TC, int EPI>
+__global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+    volume_kernel(const VolGeom<TS> geo, const KOps<N + 1, TC> ops, const Phys ph,
+                  const View<const TS> U, const Epi<TS> epi, const int64_t nbatch)
+{
+    using C = Cfg<N>;
+    // Operand lines the epilogue reads (U0, A or X of a fused stage update).
+    constexpr int NOPS = (EPI == EPI_RHS) ? 0 : (EPI == EPI_STAGE ? 1 : 2);
+    // The next-batch prefetch costs 6n registers; with operand lines in flight
+    // as well the kernel would spill, so fused stages do not prefetch.
+    constexpr bool PREFETCH = NOPS == 0;
+    constexpr int n = C::n, n2 = C::n2, n3 = C::n3, EPB = C::EPB;
+    __shared__ TC sm[EPB][S_NF][n3];
+    __shared__ double bar[EPB][2][n2];
+
+    const int tid = threadIdx.x;
+    const bool in_tile = tid < C::NT;
+    const int le = in_tile ? tid / n2 : 0;
+    const int L = in_tile ? tid - le * n2 : 0;
+
+    const TC g = (TC)ph.g, nu_h = (TC)ph.nu_h, nu_v = (TC)ph.nu_v, kT = (TC)ph.kT, kS = (TC)ph.kS;
+    const TC *__restrict__ D = ops.D;
+    const TC *__restrict__ D2 = ops.D2;
+
+    Lines<n, TC> nxt;
+    int64_t b = blockIdx.x;
+    if (PREFETCH) {
+        const int64_t e0 = b * EPB + le;
+        if (in_tile && b < nbatch && e0 < geo.E)
+            load_lines<n, TS, TC>(nxt, U, geo.h, e0, L);
+    }
+
+    for (; b < nbatch; b += gridDim.x) {
+        const int64_t e = b * EPB + le;
+        const bool act = in_tile && e < geo.E;
+        Lines<n, TC> cur;
+        if (PREFETCH)
+            cur = nxt;
+        else if (act)
+            load_lines<n, TS, TC>(cur, U, geo.h, e, L);
+
+        // Register partial tendencies of the xi-line owned in phases 1 and 4.
+        TC pHu[n], pHv[n], pHT[n], pHS[n];
+
+        // ---------------- phase 1: xi-line (j, k) = (L % n, L / n) ----------------
+        if (act) {
+            TC H[n], u[n], v[n], T[n], S[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                H[i] = cur.et[i] + cur.hb[i];
+                const TC hi = hinv_of(H[i]);
+                u[i] = cur.hu[i] * hi;
+                v[i] = cur.hv[i] * hi;
+                T[i] = cur.ht[i] * hi;
+                S[i] = cur.hs[i] * hi;
+            }
+            TC fc[n];
+            if (geo.f.p) {
+                const uint32_t fo = lane_off(geo.f, e, L);
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    fc[i] = ldo<TS, TC>(geo.f.p + i * geo.f.is, fo);
+            } else {
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    fc[i] = (TC)geo.f0;
+            }
+
+            // xi-derivatives (momentum_advection :275-305, tracer_advection
+            // :377-393, barotropic_pressure_gradient :322-337,
+            // horizontal_diffusion :339-355)
+            TC x[n], d[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = cur.hu[i] * u[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHu[i] = -d[i] + fc[i] * cur.hv[i];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = cur.hv[i] * u[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHv[i] = -d[i] + (-fc[i]) * cur.hu[i];
+            dline<n>(D, cur.et, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHu[i] += (-g * H[i]) * d[i];
+            dline<n>(D2, u, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHu[i] += H[i] * (nu_h * d[i]);
+            dline<n>(D2, v, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHv[i] += H[i] * (nu_h * d[i]);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = u[i] * cur.ht[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHT[i] = -d[i];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = u[i] * cur.hs[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHS[i] = -d[i];
+
+            // Stage the fields the eta- and zeta-line owners need.
+            TC *s = &sm[le][0][0];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int nd_ = L * n + i;
+                s[S_FUV * n3 + nd_] = cur.hu[i] * v[i];
+                s[S_FVV * n3 + nd_] = cur.hv[i] * v[i];
+                s[S_FTV * n3 + nd_] = v[i] * cur.ht[i];
+                s[S_FSV * n3 + nd_] = v[i] * cur.hs[i];
+                s[S_ETA * n3 + nd_] = cur.et[i];
+                s[S_U * n3 + nd_] = u[i];
+                s[S_V * n3 + nd_] = v[i];
+                s[S_H * n3 + nd_] = H[i];
+                s[S_T * n3 + nd_] = T[i];
+                s[S_S * n3 + nd_] = S[i];
+            }
+            // Column integrals of compute_eta_rhs (:244-254): over the xi index,
+            // summed in node order from 0, in FP64 at every precision.
+            double su = 0.0, sv = 0.0;
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                su += ops.sw[i] * (double)cur.hu[i];
+                sv += ops.sw[i] * (double)cur.hv[i];
+            }
+            bar[le][0][L] = su;
+            bar[le][1][L] = sv;
+        }
+        __syncthreads();
+
+        TC o1[5][n], o2[5][n];
+        if (PREFETCH) {
+            // Next batch of this workgroup, in flight during phases 2-4.
+            const int64_t bn = b + gridDim.x;
+            const int64_t en = bn * EPB + le;
+            if (in_tile && bn < nbatch && en < geo.E)
+                load_lines<n, TS, TC>(nxt, U, geo.h, en, L);
+        }
+
+        // ---------------- phase 2: eta-line (i, k) = (L % n, L / n) ----------------
+        if (act) {
+            const int i0 = L % n, k0 = L / n;
+            TC *s = &sm[le][0][0];
+            TC fuv[n], fvv[n], ftv[n], fsv[n], et[n], u[n], v[n], H[n];
+#pragma unroll
+            for (int j = 0; j < n; ++j) {
+                const int m = i0 + n * j + n2 * k0;
+                fuv[j] = s[S_FUV * n3 + m];
+                fvv[j] = s[S_FVV * n3 + m];
+                ftv[j] = s[S_FTV * n3 + m];
+                fsv[j] = s[S_FSV * n3 + m];
+                et[j] = s[S_ETA * n3 + m];
+                u[j] = s[S_U * n3 + m];
+                v[j] = s[S_V * n3 + m];
+                H[j] = s[S_H * n3 + m];
+            }
+            TC d[n], aHu[n], aHv[n];
+            dline<n>(D, fuv, d);
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                aHu[j] = -d[j];
+            dline<n>(D2, u, d);
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                aHu[j] += H[j] * (nu_h * d[j]);
+            dline<n>(D, fvv, d);
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                aHv[j] = -d[j];
+            dline<n>(D, et, d);
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                aHv[j] += (-g * H[j]) * d[j];
+            dline<n>(D2, v, d);
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+                aHv[j] += H[j] * (nu_h * d[j]);
+            TC dT[n], dS[n];
+            dline<n>(D, ftv, dT);
+            dline<n>(D, fsv, dS);
+#pragma unroll
+            for (int j = 0; j < n; ++j) {
+                const int m = i0 + n * j + n2 * k0;
+                s[S_FUV * n3 + m] = aHu[j];
+                s[S_FVV * n3 + m] = aHv[j];
+                s[S_FTV * n3 + m] = -dT[j];
+                s[S_FSV * n3 + m] = -dS[j];
+            }
+        }
+        __syncthreads();
+
+        // Operand lines of the fused stage update: a single array is in flight
+        // during phase 3; two arrays are loaded after phase 3, when the
+        // zeta-line temporaries are dead (loading them earlier spills).
+        if (NOPS == 1 && act) {
+            const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
+            const bool two = NOPS == 2 && !(EPI == EPI_LSRK && epi.acc_mode == 1);
+            const View<const TS> V2 =
+                (EPI == EPI_COMB) ? epi.X : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
+            const uint32_t q1 = lane_off(V1, e, L);
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    o1[f][i] = ldo<TS, TC>(V1.p + f * V1.fs + i * V1.is, q1);
+            if (two) {
+                const uint32_t q2 = lane_off(V2, e, L);
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                        o2[f][i] = ldo<TS, TC>(V2.p + f * V2.fs + i * V2.is, q2);
+            }
+        }
+
+        // ---------------- phase 3: zeta-line (i, j) = L ----------------
+        if (act) {
+            TC *s = &sm[le][0][0];
+            TC u[n], v[n], T[n], S[n], hi[n];
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                const int m = L + n2 * k;
+                u[k] = s[S_U * n3 + m];
+                v[k] = s[S_V * n3 + m];
+                T[k] = s[S_T * n3 + m];
+                S[k] = s[S_S * n3 + m];
+                hi[k] = hinv_of(s[S_H * n3 + m]);
+            }
+            // vertical_diffusion :357-375: kappa * d2/dzeta2 * Hinv
+            TC d[n];
+            dline<n>(D2, u, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_FUV * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
+            dline<n>(D2, v, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_FVV * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
+            dline<n>(D2, T, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_FTV * n3 + L + n2 * k] += kT * d[k] * hi[k];
+            dline<n>(D2, S, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_FSV * n3 + L + n2 * k] += kS * d[k] * hi[k];
+        }
+        __syncthreads();
+
+        if (NOPS == 2 && act) {
+            const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
+            const bool two = NOPS == 2 && !(EPI == EPI_LSRK && epi.acc_mode == 1);
+            const View<const TS> V2 =
+                (EPI == EPI_COMB) ? epi.X : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
+            const uint32_t q1 = lane_off(V1, e, L);
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                    o1[f][i] = ldo<TS, TC>(V1.p + f * V1.fs + i * V1.is, q1);
+            if (two) {
+                const uint32_t q2 = lane_off(V2, e, L);
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                        o2[f][i] = ldo<TS, TC>(V2.p + f * V2.fs + i * V2.is, q2);
+            }
+        }
+
+        // ---------------- phase 4: xi-line (j, k) again; write ----------------
+        if (act) {
+            const TC *s = &sm[le][0][0];
+            const int j0 = L % n, k0 = L / n;
+            // eta tendency (:256-272): -(D_x2d HU_bar + D_y2d HV_bar), where the
+            // 2D "x" index is j and the 2D "y" index is k.
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; ++m) {
+                s1 += ops.Dc[j0 * n + m] * bar[le][0][m + n * k0];
+                s2 += ops.Dc[k0 * n + m] * bar[le][1][j0 + n * m];
+            }
+            const TC deta = (TC)(-(s1 + s2));
+            TC kv[5][n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int nd_ = L * n + i;
+                kv[0][i] = pHu[i] + s[S_FUV * n3 + nd_];
+                kv[1][i] = pHv[i] + s[S_FVV * n3 + nd_];
+                kv[2][i] = deta;
+                kv[3][i] = pHT[i] + s[S_FTV * n3 + nd_];
+                kv[4][i] = pHS[i] + s[S_FSV * n3 + nd_];
+            }
+            const TC c0 = (TC)epi.c0, c1 = (TC)epi.c1, c2 = (TC)epi.c2, c3 = (TC)epi.c3;
+            if (EPI == EPI_RHS) {
+                const uint32_t ro = lane_off(epi.R, e, L);
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                        sto<TS>(epi.R.p + f * epi.R.fs + i * epi.R.is, ro, (TS)kv[f][i]);
+            } else {
+                const uint32_t yo = lane_off(epi.Y, e, L);
+                const uint32_t ao = epi.A.p ? lane_off(epi.A, e, L) : 0u;
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+#pragma unroll
+                    for (int i = 0; i < n; ++i) {
+                        const TC k = kv[f][i];
+                        TC y;
+                        if (EPI == EPI_STAGE) { // Y = U0 + c1 k ; A = k (acc_mode 1)
+                            if (epi.acc_mode == 1)
+                                sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)k);
+                            y = o1[f][i] + c1 * k;
+                        } else if (EPI == EPI_STAGE_ACC) { // A = A + c2 k ; Y = U0 + c1 k
+                            sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)(o2[f][i] + c2 * k));
+                            y = o1[f][i] + c1 * k;
+                        } else if (EPI == EPI_RK4FIN) { // Y = U0 + c1 (A + k)
+                            y = o1[f][i] + c1 * (o2[f][i] + k);
+                        } else if (EPI == EPI_COMB) { // Y = c0 U0 + c1 (X + c2 k)
+                            y = c0 * o1[f][i] + c1 * (o2[f][i] + c2 * k);
+                        } else { // EPI_LSRK: A = c2 A + c1 k ; Y = X + c3 A
+                            const TC a = (epi.acc_mode == 1) ? c1 * k : c2 * o2[f][i] + c1 * k;
+                            sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)a);
+                            y = o1[f][i] + c3 * a;
+                        }
+                        sto<TS>(epi.Y.p + f * epi.Y.fs + i * epi.Y.is, yo, (TS)y);
+                    }
+            }
+        }
+        // the next batch rewrites the LDS this batch's phase 4 reads
+        __syncthreads();
+    }
+}
+
+template <int n, class TC> KOps<n, TC> make_ops(const Ops &o)
+{
+    KOps<n, TC> r{};
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < n; ++m)
+                s += o.D[i * n + m] * o.D[m * n + j];
+            r.D[i * n + j] = (TC)o.D[i * n + j];
+            r.D2[i * n + j] = (TC)s;
+            r.Dc[i * n + j] = o.D[i * n + j];
+        }
+    for (int i = 0; i < n; ++i)
+        r.sw[i] = o.sw[i];
+    return r;
+}
+
+inline int num_cus()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            cus = p.multiProcessorCount;
+        if (cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <int N, class TS, class TC, int EPI>
+hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
+                     const Epi<TS> &epi, hipStream_t s)
+{
+    using C = Cfg<N>;
+    if (geo.E <= 0)
+        return hipSuccess;
+    const int64_t nbatch = (geo.E + C::EPB - 1) / C::EPB;
+    auto kern = volume_kernel<N, TS, TC, EPI>;
+    // resident workgroups per CU (queried once per instantiation: the launch
+    // path stays free of runtime queries)
+    static const int per_cu = [&] {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kern, C::BLOCK, 0) != hipSuccess ||
+            v < 1)
+            v = 1;
+        return v;
+    }();
+    // persistent workgroups: a few rounds per workgroup so the next batch's
+    // loads overlap the current batch's arithmetic
+    int64_t grid = (int64_t)num_cus() * per_cu;
+    if (grid > nbatch)
+        grid = nbatch;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(C::BLOCK), 0, s, geo,
+                       make_ops<N + 1, TC>(ops), ph, U, epi, nbatch);
+    return hipGetLastError();
+}
+
+template <int N, class TS, class TC>
+hipError_t launch_n(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
+                    const Epi<TS> &epi, hipStream_t s)
+{
+    switch (epi.mode) {
+    case EPI_RHS: return launch_ne<N, TS, TC, EPI_RHS>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE: return launch_ne<N, TS, TC, EPI_STAGE>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE_ACC: return launch_ne<N, TS, TC, EPI_STAGE_ACC>(geo, ops, ph, U, epi, s);
+    case EPI_RK4FIN: return launch_ne<N, TS, TC, EPI_RK4FIN>(geo, ops, ph, U, epi, s);
+    case EPI_COMB: return launch_ne<N, TS, TC, EPI_COMB>(geo, ops, ph, U, epi, s);
+    case EPI_LSRK: return launch_ne<N, TS, TC, EPI_LSRK>(geo, ops, ph, U, epi, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class TS, class TC>
+hipError_t dispatch(int order, const VolGeom<TS> &geo, const Ops &ops, const Phys &ph,
+                    View<const TS> U, const Epi<TS> &epi, hipStream_t s)
+{
+    switch (order) {
+    case 1: return launch_n<1, TS, TC>(geo, ops, ph, U, epi, s);
+    case 2: return launch_n<2, TS, TC>(geo, ops, ph, U, epi, s);
+    case 3: return launch_n<3, TS, TC>(geo, ops, ph, U, epi, s);
+    case 4: return launch_n<4, TS, TC>(geo, ops, ph, U, epi, s);
+    case 5: return launch_n<5, TS, TC>(geo, ops, ph, U, epi, s);
+    case 6: return launch_n<6, TS, TC>(geo, ops, ph, U, epi, s);
+    case 7: return launch_n<7, TS, TC>(geo, ops, ph, U, epi, s);
+    case 8: return launch_n<8, TS, TC>(geo, ops, ph, U, epi, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace vol
+} // namespace drf

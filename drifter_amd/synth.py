@@ -196,6 +196,18 @@ def config2():
     return make_case(64, 64, 8, 4, 100000.0, "shelf")
 
 
+def to_lines(U, E, n):
+    """packed [e][f][i + n L] -> LINES [f][i][e][L]"""
+    nd = n ** 3
+    P = U.reshape(E, 5, n * n, n)  # e, f, L, i
+    return np.ascontiguousarray(P.transpose(1, 3, 0, 2).reshape(-1))
+
+
+def from_lines(V, E, n):
+    P = V.reshape(5, n, E, n * n)  # f, i, e, L
+    return np.ascontiguousarray(P.transpose(2, 0, 3, 1).reshape(-1))
+
+
 def to_soa(U, E, nd):
     return np.ascontiguousarray(U.reshape(E, 5, nd).transpose(1, 0, 2).reshape(-1))
 

@@ -11,6 +11,9 @@
  *   DRF_LAYOUT_PACKED  U[e][field][d]   -- the reference's packed vector
  *                                          (src/solver/simulation_driver.cpp:199-213)
  *   DRF_LAYOUT_SOA     U[field][e][d]   -- structure of arrays
+ *   DRF_LAYOUT_LINES   U[field][i][e][L] with d = i + n L -- structure of
+ *                      arrays with the xi index outermost: the kernels'
+ *                      native layout, fully coalesced loads and stores
  *   fields: 0 Hu, 1 Hv, 2 eta, 3 HT, 4 HS
  *   full state (mode splitting): [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d]
  *
@@ -44,6 +47,7 @@ typedef int drf_status;
 
 #define DRF_LAYOUT_PACKED 0
 #define DRF_LAYOUT_SOA 1
+#define DRF_LAYOUT_LINES 2
 
 #define DRF_PREC_FP64 0  /* FP64 storage and arithmetic (parity path) */
 #define DRF_PREC_MIXED 1 /* FP32 volume arithmetic and storage; FP64 column integrals */
@@ -140,6 +144,50 @@ double drf_compute_dt(double max_wave_speed, double dx_min, double cfl, double d
 drf_status drf_mode_split_step(drf_ctx *ctx, double dt_3d, int n_sub, double *full);
 drf_status drf_mode_split_step_dev(drf_ctx *ctx, double dt_3d, int n_sub, void *full,
                                    void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Octree mesh and face connectivity (OctreeAdapter,                   */
+/* include/mesh/octree_adapter.hpp:103-273)                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct drf_mesh drf_mesh;
+
+/* FaceConnectionType, include/dg/face_connection.hpp:15-46 */
+#define DRF_CONN_SAME_LEVEL 0
+#define DRF_CONN_FINE_2X1 1
+#define DRF_CONN_FINE_1X2 2
+#define DRF_CONN_FINE_3_2PLUS1 3
+#define DRF_CONN_FINE_3_1PLUS2 4
+#define DRF_CONN_FINE_2X2 5
+#define DRF_CONN_BOUNDARY 6
+
+/* OctreeAdapter(xmin, xmax, ymin, ymax, zmin, zmax), octree_adapter.cpp:14-21 */
+drf_status drf_mesh_create(double xmin, double xmax, double ymin, double ymax, double zmin,
+                           double zmax, drf_mesh **out);
+void drf_mesh_destroy(drf_mesh *mesh);
+/* OctreeAdapter::build_uniform, octree_adapter.cpp:23-84 */
+drf_status drf_mesh_build_uniform(drf_mesh *mesh, int nx, int ny, int nz);
+/* OctreeAdapter::refine (+ balance), octree_adapter.cpp:184-203; masks are
+ * RefineMask bits (1 X, 2 Y, 4 Z), include/mesh/refine_mask.hpp */
+drf_status drf_mesh_refine(drf_mesh *mesh, int64_t count, const int64_t *elems,
+                           const uint8_t *masks);
+/* OctreeAdapter::balance, octree_adapter.cpp:138-182 */
+drf_status drf_mesh_balance(drf_mesh *mesh);
+/* OctreeAdapter::num_elements */
+int64_t drf_mesh_num_elements(const drf_mesh *mesh);
+/* element bounds (E*6: xmin xmax ymin ymax zmin zmax), directional levels
+ * (E*3) and Morton codes (E) in element order; any output may be NULL */
+drf_status drf_mesh_elements(const drf_mesh *mesh, double *bounds, int32_t *levels,
+                             uint64_t *morton);
+/* OctreeAdapter::element_bounds (octree_adapter.cpp:234-239); DRF_ERR_RANGE
+ * for an invalid element (std::out_of_range in the reference) */
+drf_status drf_mesh_element_bounds(const drf_mesh *mesh, int64_t elem, double *bounds6);
+/* OctreeAdapter::build_face_connections (octree_adapter.cpp:369-389) as flat
+ * arrays over (element, face): type[E*6], count[E*6], and up to four
+ * neighbours elems/faces/subs[E*6*4] (FaceConnection::fine_elems,
+ * fine_face_ids, subface_indices; unused slots -1) */
+drf_status drf_mesh_face_connections(const drf_mesh *mesh, uint8_t *type, int8_t *count,
+                                     int64_t *elems, int8_t *faces, int8_t *subs);
 
 /* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */

@@ -1,0 +1,193 @@
+// Header-only C++ facade over the drf C ABI, shaped like the reference's
+// solver interfaces so that a C++ time integrator or mode splitter can call
+// the MI355X right-hand side in place of the CPU operators:
+//   PrimitiveEquationsSolver  include/physics/primitive_equations.hpp:217-265
+//   RHSFunction / TimeStepper include/solver/time_stepper.hpp:10-33
+//   ModeSplittingSolver       include/physics/mode_splitting.hpp:203-266
+//   AdaptiveTimeController    include/solver/time_stepper.hpp:64-81
+// Vectors are std::vector<double> (Eigen's VecX maps onto them through
+// .data()); failures throw std::runtime_error carrying drf_last_error(), as
+// the reference reports errors by exception.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "drf.h"
+
+namespace drifter_gpu {
+
+using Real = double;
+using Vec = std::vector<double>;
+// RHSFunction, include/solver/time_stepper.hpp:10
+using RHSFunction = std::function<void(Real t, const Vec &U, Vec &dUdt)>;
+
+class Error : public std::runtime_error {
+  public:
+    Error(drf_status s, const std::string &what)
+        : std::runtime_error("drf status " + std::to_string(s) + ": " + what), status(s)
+    {
+    }
+    drf_status status;
+};
+
+inline void check(drf_status s)
+{
+    if (s != DRF_OK)
+        throw Error(s, drf_last_error());
+}
+
+// CoriolisParameter, include/physics/primitive_equations.hpp:52-79
+struct CoriolisParameter {
+    Real f0 = 1e-4, beta = 2e-11, y0 = 0.0;
+    Real operator()(Real y) const { return f0 + beta * (y - y0); }
+};
+
+class PrimitiveEquationsSolver {
+  public:
+    explicit PrimitiveEquationsSolver(int order, int precision = DRF_PREC_FP64, int device = 0)
+        : order_(order)
+    {
+        drf_ctx *c = nullptr;
+        check(drf_create(order, precision, device, &c));
+        ctx_.reset(c);
+    }
+
+    int order() const { return order_; }
+    int num_dofs() const { return (order_ + 1) * (order_ + 1) * (order_ + 1); }
+    drf_ctx *ctx() const { return ctx_.get(); }
+
+    void set_physics(const drf_physics &p) { check(drf_set_physics(ctx(), &p)); }
+
+    // PrimitiveEquationsSolver::initialize (:229-231): per-element vectors
+    void initialize(int num_elements, const std::vector<Vec> &bathymetry,
+                    const std::vector<Vec> &dh_dx, const std::vector<Vec> &dh_dy,
+                    const CoriolisParameter &coriolis, const std::vector<Vec> &y_positions)
+    {
+        (void)dh_dx;
+        (void)dh_dy;
+        const int nd = num_dofs();
+        Vec h, y;
+        h.reserve((size_t)num_elements * nd);
+        for (int e = 0; e < num_elements; ++e) {
+            if ((int)bathymetry.at(e).size() != nd)
+                throw std::invalid_argument("bathymetry must hold ndof values per element");
+            h.insert(h.end(), bathymetry[e].begin(), bathymetry[e].end());
+        }
+        if (!y_positions.empty())
+            for (int e = 0; e < num_elements; ++e)
+                y.insert(y.end(), y_positions.at(e).begin(), y_positions.at(e).end());
+        check(drf_initialize(ctx(), num_elements, h.data(), nullptr, nullptr, coriolis.f0,
+                             coriolis.beta, coriolis.y0, y.empty() ? nullptr : y.data()));
+        num_elements_ = num_elements;
+    }
+
+    // SimulationDriver::compute_rhs on the packed vector (simulation_driver.cpp:170-197)
+    void compute_rhs(Real t, const Vec &U, Vec &dUdt) const
+    {
+        dUdt.resize(U.size());
+        check(drf_rhs(ctx(), t, U.data(), dUdt.data()));
+    }
+
+    RHSFunction rhs_function() const
+    {
+        return [this](Real t, const Vec &U, Vec &dUdt) { compute_rhs(t, U, dUdt); };
+    }
+
+    Real max_wave_speed(const Vec &U) const
+    {
+        Real c = 0.0;
+        check(drf_max_wave_speed(ctx(), U.data(), &c));
+        return c;
+    }
+
+    int num_elements() const { return num_elements_; }
+
+  private:
+    struct Del {
+        void operator()(drf_ctx *c) const { drf_destroy(c); }
+    };
+    int order_;
+    int num_elements_ = 0;
+    std::unique_ptr<drf_ctx, Del> ctx_;
+};
+
+// TimeStepper with the reference's factories; step() runs fused on the GPU
+// through the solver that owns the right-hand side.
+class TimeStepper {
+  public:
+    static std::unique_ptr<TimeStepper> forward_euler() { return make(DRF_RK_EULER, 1, 1); }
+    static std::unique_ptr<TimeStepper> rk2_midpoint() { return make(DRF_RK_RK2, 2, 2); }
+    static std::unique_ptr<TimeStepper> rk3_ssp() { return make(DRF_RK_SSPRK3, 3, 3); }
+    static std::unique_ptr<TimeStepper> rk4_classic() { return make(DRF_RK_RK4, 4, 4); }
+    static std::unique_ptr<TimeStepper> lsrk45() { return make(DRF_RK_LSRK45, 4, 5); }
+    static std::unique_ptr<TimeStepper> rk45_dormand_prince() { return make(DRF_RK_DP45, 5, 6); }
+    // RKDG order selection, src/solver/time_stepper.cpp:209-220
+    static std::unique_ptr<TimeStepper> rkdg(int order)
+    {
+        if (order <= 1)
+            return forward_euler();
+        if (order == 2)
+            return rk2_midpoint();
+        if (order == 3)
+            return rk3_ssp();
+        return rk4_classic();
+    }
+
+    void step(Real t, Real dt, Vec &U, const PrimitiveEquationsSolver &solver) const
+    {
+        check(drf_rk_step(solver.ctx(), scheme_, t, dt, U.data()));
+    }
+    int order() const { return order_; }
+    int num_stages() const { return stages_; }
+    int scheme() const { return scheme_; }
+
+  private:
+    TimeStepper(int scheme, int order, int stages) : scheme_(scheme), order_(order), stages_(stages)
+    {
+    }
+    static std::unique_ptr<TimeStepper> make(int s, int o, int st)
+    {
+        return std::unique_ptr<TimeStepper>(new TimeStepper(s, o, st));
+    }
+    int scheme_, order_, stages_;
+};
+
+// AdaptiveTimeController::compute_dt (src/solver/time_stepper.cpp:261-268)
+class AdaptiveTimeController {
+  public:
+    explicit AdaptiveTimeController(Real cfl = 0.5, Real dt_min = 1.0e-10, Real dt_max = 1.0e6)
+        : cfl_(cfl), dt_min_(dt_min), dt_max_(dt_max)
+    {
+    }
+    Real compute_dt(Real max_wave_speed, Real min_element_size) const
+    {
+        return drf_compute_dt(max_wave_speed, min_element_size, cfl_, dt_min_, dt_max_);
+    }
+
+  private:
+    Real cfl_, dt_min_, dt_max_;
+};
+
+// ModeSplittingSolver::step on the flat full state
+// [elem][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][ndof].
+class ModeSplittingSolver {
+  public:
+    ModeSplittingSolver(const PrimitiveEquationsSolver &solver, int subcycles = 30)
+        : solver_(solver), subcycles_(subcycles)
+    {
+    }
+    void step(Real dt_3d, Vec &full_state) const
+    {
+        check(drf_mode_split_step(solver_.ctx(), dt_3d, subcycles_, full_state.data()));
+    }
+
+  private:
+    const PrimitiveEquationsSolver &solver_;
+    int subcycles_;
+};
+
+} // namespace drifter_gpu

@@ -85,6 +85,28 @@ double orc_compute_dt(double max_wave_speed, double dx_min, double cfl, double d
 int orc_mode_split_step(int order, int64_t n_elem, double dt_3d, int n_sub, double *full,
                         const double *h, const double *f, const orc_physics *phys);
 
+/* ---- A17: octree and face connectivity (src/mesh/octree_adapter.cpp) ---- */
+/* FaceConnectionType order, include/dg/face_connection.hpp:15-46 */
+enum {
+    ORC_CONN_SAME = 0, ORC_CONN_FINE2X1 = 1, ORC_CONN_FINE1X2 = 2, ORC_CONN_FINE3_2P1 = 3,
+    ORC_CONN_FINE3_1P2 = 4, ORC_CONN_FINE2X2 = 5, ORC_CONN_BOUNDARY = 6
+};
+typedef struct orc_mesh orc_mesh;
+orc_mesh *orc_mesh_create(double xmin, double xmax, double ymin, double ymax, double zmin,
+                          double zmax);
+void orc_mesh_destroy(orc_mesh *m);
+void orc_mesh_build_uniform(orc_mesh *m, int nx, int ny, int nz);
+/* refine leaves elems[i] with masks[i] (bit0 X, bit1 Y, bit2 Z), then balance */
+void orc_mesh_refine(orc_mesh *m, int64_t count, const int64_t *elems, const int *masks);
+void orc_mesh_balance(orc_mesh *m);
+int64_t orc_mesh_num_elements(const orc_mesh *m);
+/* bounds E*6, levels E*3, morton E (any may be NULL) */
+void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t *morton);
+/* per (element, face): type, neighbour count, up to 4 neighbours with their
+ * face ids and sub-face indices (unused slots -1) */
+void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
+                          int8_t *faces, int8_t *subs);
+
 #ifdef __cplusplus
 }
 #endif

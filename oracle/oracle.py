@@ -53,6 +53,14 @@ def load():
         "orc_compute_dt": (C.c_double, [C.c_double] * 5),
         "orc_mode_split_step": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_int, _PD, _PD, _PD,
                                           C.POINTER(OrcPhysics)]),
+        "orc_mesh_create": (C.c_void_p, [C.c_double] * 6),
+        "orc_mesh_destroy": (None, [C.c_void_p]),
+        "orc_mesh_build_uniform": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+        "orc_mesh_refine": (None, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+        "orc_mesh_balance": (None, [C.c_void_p]),
+        "orc_mesh_num_elements": (C.c_int64, [C.c_void_p]),
+        "orc_mesh_elements": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_mesh_connections": (None, [C.c_void_p] * 6),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -159,3 +167,54 @@ def mode_split_step(order, dt3, nsub, full, h, f, phys=None):
     p = phys or physics()
     assert load().orc_mode_split_step(order, E, dt3, nsub, _p(V), _p(h), _p(f), C.byref(p)) == 0
     return V
+
+
+CONN_SAME, CONN_FINE2X1, CONN_FINE1X2, CONN_FINE3_2P1, CONN_FINE3_1P2, CONN_FINE2X2, CONN_BOUNDARY = range(7)
+
+
+class Mesh:
+    """OctreeAdapter restatement (oracle/mesh.c)."""
+
+    def __init__(self, xmin, xmax, ymin, ymax, zmin, zmax):
+        self._l = load()
+        self.h = self._l.orc_mesh_create(xmin, xmax, ymin, ymax, zmin, zmax)
+
+    def __del__(self):
+        try:
+            self._l.orc_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def build_uniform(self, nx, ny, nz):
+        self._l.orc_mesh_build_uniform(self.h, nx, ny, nz)
+
+    def refine(self, elems, masks):
+        e = np.ascontiguousarray(elems, dtype=np.int64)
+        m = np.ascontiguousarray(masks, dtype=np.int32)
+        self._l.orc_mesh_refine(self.h, e.size, e.ctypes.data, m.ctypes.data)
+
+    def balance(self):
+        self._l.orc_mesh_balance(self.h)
+
+    @property
+    def num_elements(self):
+        return self._l.orc_mesh_num_elements(self.h)
+
+    def elements(self):
+        E = self.num_elements
+        b = np.zeros((E, 6))
+        lv = np.zeros((E, 3), dtype=np.int32)
+        mo = np.zeros(E, dtype=np.uint64)
+        self._l.orc_mesh_elements(self.h, b.ctypes.data, lv.ctypes.data, mo.ctypes.data)
+        return b, lv, mo
+
+    def connections(self):
+        E = self.num_elements
+        t = np.zeros((E, 6), dtype=np.uint8)
+        c = np.zeros((E, 6), dtype=np.int8)
+        el = np.zeros((E, 6, 4), dtype=np.int64)
+        fa = np.zeros((E, 6, 4), dtype=np.int8)
+        su = np.zeros((E, 6, 4), dtype=np.int8)
+        self._l.orc_mesh_connections(self.h, t.ctypes.data, c.ctypes.data, el.ctypes.data,
+                                     fa.ctypes.data, su.ctypes.data)
+        return t, c, el, fa, su

@@ -63,7 +63,7 @@ def test_ragged_last_block(orc):
 
 
 def test_layouts_agree_bitwise_and_full_config2_sampled(orc):
-    """64x64x8, N=4 (32,768 elements) on device in SoA and packed layouts;
+    """64x64x8, N=4 (32,768 elements) on device in the packed, SoA and LINES layouts;
     every element is checked against the oracle on a sample (the RHS is
     element-local, so a sample of elements is a self-contained input)."""
     import torch
@@ -76,14 +76,19 @@ def test_layouts_agree_bitwise_and_full_config2_sampled(orc):
     dev = torch.device("cuda", 0)
     Up = torch.from_numpy(case.U).to(dev)
     Us = torch.from_numpy(synth.to_soa(case.U, E, nd)).to(dev)
+    Ul = torch.from_numpy(synth.to_lines(case.U, E, 5)).to(dev)
     Rp = torch.empty_like(Up)
     Rs = torch.empty_like(Us)
+    Rl = torch.empty_like(Ul)
     s.compute_rhs_dev(Up, Rp, _lib.LAYOUT_PACKED)
     s.compute_rhs_dev(Us, Rs, _lib.LAYOUT_SOA)
+    s.compute_rhs_dev(Ul, Rl, _lib.LAYOUT_LINES)
     torch.cuda.synchronize()
     rp = Rp.cpu().numpy()
     rs = synth.from_soa(Rs.cpu().numpy(), E, nd)
+    rl = synth.from_lines(Rl.cpu().numpy(), E, 5)
     assert np.array_equal(rp, rs)
+    assert np.array_equal(rp, rl)
     assert np.all(np.isfinite(rp))
     rng = np.random.default_rng(0)
     idx = np.sort(rng.choice(E, 400, replace=False))

@@ -39,19 +39,26 @@ def _driver_dt(orc, case):
 
 @pytest.mark.parametrize("scheme", range(6))
 @pytest.mark.parametrize("fused", [True, False])
-def test_rk_step_matches_oracle(orc, scheme, fused):
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_SOA, _lib.LAYOUT_LINES])
+def test_rk_step_matches_oracle(orc, scheme, fused, layout):
     import torch
 
     case = synth.config1()
     dt, _ = _driver_dt(orc, case)
-    print(f"[rk] scheme={scheme} fused={fused} dt={dt:.4e}", flush=True)
+    print(f"[rk] scheme={scheme} fused={fused} layout={layout} dt={dt:.4e}", flush=True)
     ref = orc.rk_step(scheme, 3, 0.0, dt, case.U, case.h, np.full(case.h.size, 1e-4))
     s = PrimitiveEquationsSolver(3)
     s.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U = torch.from_numpy(synth.to_soa(case.U, case.E, case.nd)).cuda()
-    s.rk_step_dev(scheme, 0.0, dt, U, _lib.LAYOUT_SOA, fused=fused)
+    if layout == _lib.LAYOUT_SOA:
+        U = torch.from_numpy(synth.to_soa(case.U, case.E, case.nd)).cuda()
+    else:
+        U = torch.from_numpy(synth.to_lines(case.U, case.E, case.n)).cuda()
+    s.rk_step_dev(scheme, 0.0, dt, U, layout, fused=fused)
     torch.cuda.synchronize()
-    got = synth.from_soa(U.cpu().numpy(), case.E, case.nd)
+    if layout == _lib.LAYOUT_SOA:
+        got = synth.from_soa(U.cpu().numpy(), case.E, case.nd)
+    else:
+        got = synth.from_lines(U.cpu().numpy(), case.E, case.n)
     errs = rel_l2(got, ref, case.E, 5, case.nd)
     print(f"  rel L2: {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) <= TOL
@@ -80,6 +87,8 @@ def test_wave_speed_bit_exact(orc):
         assert s.max_wave_speed(case.U) == ref
         Ud = torch.from_numpy(synth.to_soa(case.U, case.E, case.nd)).cuda()
         assert s.max_wave_speed_dev(Ud, _lib.LAYOUT_SOA) == ref
+        Ul = torch.from_numpy(synth.to_lines(case.U, case.E, case.n)).cuda()
+        assert s.max_wave_speed_dev(Ul, _lib.LAYOUT_LINES) == ref
         Hd, ud, vd = (torch.from_numpy(a).cuda() for a in (H, u, v))
         assert s.max_wave_speed_derived_dev(Hd, ud, vd, case.nd) == ref
         s.close()
