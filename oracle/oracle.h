@@ -107,6 +107,31 @@ void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t 
 void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
                           int8_t *faces, int8_t *subs);
 
+/* ---- A14-A16, A18: fluxes, conforming interfaces, mortars ---- */
+enum {
+    ORC_FLUX_SW_LF = 0,      /* factory shallow_water_lax_friedrichs, p = {g} */
+    ORC_FLUX_SW_HLLC = 1,    /* ShallowWaterHLLCFlux, p = {g} */
+    ORC_FLUX_SW_ROE = 2,     /* ShallowWaterRoeFlux, p = {g, entropy_fix} */
+    ORC_FLUX_SW_CENTRAL = 3, /* CentralFlux with the shallow-water flux, p = {g} */
+    ORC_FLUX_PE_LF5 = 4,     /* apply_interface_fluxes lambda, p = {g} */
+    ORC_FLUX_UPWIND = 5,     /* UpwindFlux, p = {vx, vy, vz} */
+    ORC_FLUX_LF_ADV = 6      /* LaxFriedrichsFlux for F(U) = v U, p = {vx, vy, vz, lambda} */
+};
+/* returns the number of variables written to out */
+int orc_flux(int kind, const double *UL, const double *UR, const double *n, const double *p,
+             double *out);
+/* face DOF it + n jt -> volume DOF index (LGL face extraction) */
+void orc_face_map(int order, int face, int *vol_index);
+/* ConformingInterface::compute_flux for a scalar flux; returns #quad points */
+int orc_conforming_flux(int order, int kind, const double *p, const double *UL,
+                        const double *UR, const double *n, double *rhs_l, double *rhs_r);
+/* MortarSpace operators (row-major): M, Minv, Pc, Lc, Pf[f], Lf[f]; returns #fine */
+int orc_mortar_operators(int order, int type, double *M, double *Minv, double *Pc, double *Lc,
+                         double *Pf, double *Lf);
+/* MortarSpace::compute_mortar_flux (scalar); Uf / rhs_f are nfine*(N+1)^2 */
+int orc_mortar_flux(int order, int type, int kind, const double *p, const double *Uc,
+                    const double *Uf, const double *n, double *rhs_c, double *rhs_f);
+
 #ifdef __cplusplus
 }
 #endif

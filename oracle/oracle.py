@@ -61,6 +61,11 @@ def load():
         "orc_mesh_num_elements": (C.c_int64, [C.c_void_p]),
         "orc_mesh_elements": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_mesh_connections": (None, [C.c_void_p] * 6),
+        "orc_flux": (C.c_int, [C.c_int, _PD, _PD, _PD, _PD, _PD]),
+        "orc_face_map": (None, [C.c_int, C.c_int, C.c_void_p]),
+        "orc_conforming_flux": (C.c_int, [C.c_int, C.c_int, _PD, _PD, _PD, _PD, _PD, _PD]),
+        "orc_mortar_operators": (C.c_int, [C.c_int, C.c_int] + [_PD] * 6),
+        "orc_mortar_flux": (C.c_int, [C.c_int, C.c_int, C.c_int, _PD, _PD, _PD, _PD, _PD, _PD]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -218,3 +223,53 @@ class Mesh:
         self._l.orc_mesh_connections(self.h, t.ctypes.data, c.ctypes.data, el.ctypes.data,
                                      fa.ctypes.data, su.ctypes.data)
         return t, c, el, fa, su
+
+
+SW_LF, SW_HLLC, SW_ROE, SW_CENTRAL, PE_LF5, UPWIND, LF_ADV = range(7)
+
+
+def _v(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def flux(kind, UL, UR, n, p):
+    out = np.zeros(5)
+    nv = load().orc_flux(kind, _p(_v(UL)), _p(_v(UR)), _p(_v(n)), _p(_v(p)), _p(out))
+    assert nv > 0
+    return out[:nv]
+
+
+def face_map(order, face):
+    m = np.zeros((order + 1) ** 2, dtype=np.int32)
+    load().orc_face_map(order, face, m.ctypes.data)
+    return m
+
+
+def conforming_flux(order, kind, p, UL, UR, n):
+    nf = (order + 1) ** 2
+    rl = np.zeros(nf)
+    rr = np.zeros(nf)
+    load().orc_conforming_flux(order, kind, _p(_v(p)), _p(_v(UL)), _p(_v(UR)), _p(_v(n)),
+                               _p(rl), _p(rr))
+    return rl, rr
+
+
+def mortar_operators(order, ctype):
+    nm = (order + 1) ** 2
+    M = np.zeros((nm, nm))
+    Mi = np.zeros((nm, nm))
+    Pc = np.zeros((nm, nm))
+    Lc = np.zeros((nm, nm))
+    Pf = np.zeros((4, nm, nm))
+    Lf = np.zeros((4, nm, nm))
+    nfine = load().orc_mortar_operators(order, ctype, _p(M), _p(Mi), _p(Pc), _p(Lc), _p(Pf), _p(Lf))
+    return M, Mi, Pc, Lc, Pf[:nfine], Lf[:nfine]
+
+
+def mortar_flux(order, ctype, kind, p, Uc, Uf, n):
+    nm = (order + 1) ** 2
+    rc = np.zeros(nm)
+    rf = np.zeros(4 * nm)
+    nfine = load().orc_mortar_flux(order, ctype, kind, _p(_v(p)), _p(_v(Uc)), _p(_v(Uf).reshape(-1)),
+                                   _p(_v(n)), _p(rc), _p(rf))
+    return rc, rf[:nfine * nm].reshape(nfine, nm)
