@@ -83,18 +83,52 @@ def cpu_baseline(case, target_s=12.0):
     }
 
 
+def copy_probe(steps, warmup):
+    """Plain device copies with the RHS's field sizes (64x64x8 elements,
+    125 nodes, FP64): (a) 5 fields copied, 1 read + 1 write per value;
+    (b) 5 fields plus a broadcast 6th field added into 5 outputs, the volume
+    kernel's 6-in/5-out pattern.  Their HBM counters per dispatch give the
+    scale of FETCH_SIZE / WRITE_SIZE, and their time the copy ceiling."""
+    import torch
+
+    nn = 64 * 64 * 8 * 125
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    src = torch.ones(6 * nn, dtype=torch.float64, device=dev)
+    dst = torch.empty(5 * nn, dtype=torch.float64, device=dev)
+    a = time_launches(lambda: dst.copy_(src[: 5 * nn]), steps, warmup, stream)
+    b = time_launches(lambda: torch.add(src[: 5 * nn].view(5, nn), src[5 * nn:].view(1, nn),
+                                        out=dst.view(5, nn)), steps, warmup, stream)
+    ba, bb = 10 * nn * 8, 11 * nn * 8
+    print(json.dumps({
+        "copy_5_fields": {"us": round(a * 1e3, 2), "bytes": ba,
+                          "gb_s": round(ba / (a * 1e-3) / 1e9, 1),
+                          "read_bytes": 5 * nn * 8, "write_bytes": 5 * nn * 8},
+        "add_6_in_5_out": {"us": round(b * 1e3, 2), "bytes": bb,
+                           "gb_s": round(bb / (b * 1e-3) / 1e9, 1),
+                           "read_bytes": 6 * nn * 8, "write_bytes": 5 * nn * 8},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time plain copies of known size instead (HBM counter calibration "
+                         "and copy ceiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards by "
-              "contiguous Morton element ranges with no exchange (see DESIGN.md).",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the element-local "
+              "RHS shards by contiguous Morton element ranges with no exchange, and no "
+              "multi-GPU figure is measured here.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.steps, args.warmup)
+        return
 
     import numpy as np
     import torch

@@ -63,6 +63,16 @@ SIGNATURES = {
     "drf_mesh_elements": (_I, [_P, _P, _P, _P]),
     "drf_mesh_element_bounds": (_I, [_P, _L, _PD]),
     "drf_mesh_face_connections": (_I, [_P, _P, _P, _P, _P, _P]),
+    "drf_flux_points": (_I, [_I, _L, _P, _P, _P, _PD, _P, _P]),
+    "drf_face_map": (_I, [_I, _I, _P]),
+    "drf_gather_face_traces": (_I, [_I, _L, _L, _L, _P, _P, _P, _P, _P]),
+    "drf_conforming_flux_batch": (_I, [_I, _I, _PD, _L, _P, _P, _P, _P, _P, _P]),
+    "drf_mortar_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "drf_mortar_destroy": (None, [_P]),
+    "drf_mortar_num_fine": (_I, [_P]),
+    "drf_mortar_num_dofs": (_I, [_P]),
+    "drf_mortar_operators": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "drf_mortar_flux_batch": (_I, [_P, _I, _PD, _L, _P, _P, _P, _P, _P, _P]),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

@@ -73,6 +73,49 @@ int lgl_nodes(int n, double *x, double *w)
     return 0;
 }
 
+// Gauss-Legendre nodes: Newton on P_n from x = -cos((2i+1) pi / 2n), the
+// same iteration and stopping rule as the reference's GaussQuadrature1D
+// (src/dg/quadrature_3d.cpp:13-31) via compute_gauss_legendre_nodes
+// (src/dg/basis_hexahedron.cpp:14-64); weights 2 / ((1 - x^2) P_n'(x)^2).
+int gl_nodes(int n, double *x, double *w)
+{
+    if (n < 1 || n > 2 * kMaxN)
+        return -1;
+    if (n == 1) {
+        x[0] = 0.0;
+        w[0] = 2.0;
+        return 0;
+    }
+    if (n == 2) {
+        x[0] = -std::sqrt(1.0 / 3.0);
+        x[1] = -x[0];
+        w[0] = w[1] = 1.0;
+        return 0;
+    }
+    Leg tab[2 * kMaxN + 2];
+    for (int i = 0; i <= (n - 1) / 2; ++i) {
+        double xi = -std::cos((2.0 * i + 1.0) / (2.0 * n) * M_PI);
+        for (int it = 0; it < 100; ++it) {
+            const double prev = xi;
+            legendre_table(n, xi, tab);
+            xi = xi - tab[n].p / tab[n].dp;
+            if (std::fabs(xi - prev) < 1e-15)
+                break;
+        }
+        legendre_table(n, xi, tab);
+        const double wi = 2.0 / ((1.0 - xi * xi) * tab[n].dp * tab[n].dp);
+        x[i] = xi;
+        x[n - 1 - i] = -xi;
+        w[i] = w[n - 1 - i] = wi;
+    }
+    if (n % 2 == 1) {
+        legendre_table(n, 0.0, tab);
+        x[n / 2] = 0.0;
+        w[n / 2] = 2.0 / (tab[n].dp * tab[n].dp);
+    }
+    return 0;
+}
+
 void derivative_matrix(int n, const double *x, double *D)
 {
     double bw[kMaxN];

@@ -190,6 +190,72 @@ drf_status drf_mesh_face_connections(const drf_mesh *mesh, uint8_t *type, int8_t
                                      int64_t *elems, int8_t *faces, int8_t *subs);
 
 /* ------------------------------------------------------------------ */
+/* Numerical fluxes (NumericalFlux::flux, include/flux/numerical_flux.hpp:41)
+ * over a batch of face points                                         */
+/* ------------------------------------------------------------------ */
+
+/* kind                      reference                                 params      vars */
+#define DRF_FLUX_SW_LF 0      /* shallow_water_lax_friedrichs, numerical_flux.cpp:134-181; {g}       3 */
+#define DRF_FLUX_SW_HLLC 1    /* ShallowWaterHLLCFlux, numerical_flux.hpp:110-131; {g}                3 */
+#define DRF_FLUX_SW_ROE 2     /* ShallowWaterRoeFlux, numerical_flux.hpp:134-150; {g, entropy_fix}   3 */
+#define DRF_FLUX_SW_CENTRAL 3 /* CentralFlux with the shallow-water flux, :78-89; {g}                3 */
+#define DRF_FLUX_PE_LF5 4     /* apply_interface_fluxes, primitive_equations.cpp:452-492; {g}        5 */
+#define DRF_FLUX_UPWIND 5     /* UpwindFlux, numerical_flux.hpp:92-107; {vx, vy, vz}                 1 */
+#define DRF_FLUX_LF_ADV 6     /* LaxFriedrichsFlux of F(U) = v U, :57-75; {vx, vy, vz, lambda}       1 */
+
+/* Replaces flux(U_L, U_R, n) evaluated point by point.  Device pointers,
+ * structure of arrays: UL, UR, out are [var][npts], normals [3][npts].
+ * params: host array of up to 4 doubles (table above). */
+drf_status drf_flux_points(int kind, int64_t npts, const void *UL, const void *UR,
+                           const void *normals, const double *params, void *out, void *stream);
+
+/* Face DOF it + n jt of face 0..5 (-xi,+xi,-eta,+eta,-zeta,+zeta) <- volume
+ * node vol_index[it + n jt] (LGL face extraction,
+ * src/dg/basis_hexahedron.cpp:431-522).  Host call; vol_index holds (N+1)^2. */
+drf_status drf_face_map(int order, int face, int32_t *vol_index);
+
+/* Face traces of one nodal field for a batch of (element, face) pairs:
+ * out[k][q] = field[elems[k] * elem_stride + vol_index(faces[k])[q]],
+ * out is [nfaces][(N+1)^2].  elem_stride = (N+1)^3 for one field [e][d];
+ * (number of fields) (N+1)^3 for a field of the packed layout.  An element
+ * outside [0, n_elem) or a face outside 0..5 gives NaN for that face.  Device
+ * pointers (elems int64, faces int8). */
+drf_status drf_gather_face_traces(int order, int64_t n_elem, int64_t elem_stride, int64_t nfaces,
+                                  const void *elems, const void *faces, const void *field,
+                                  void *out, void *stream);
+
+/* ConformingInterface::compute_flux (src/dg/mortar.cpp:320-343) for a batch
+ * of faces with a scalar flux (DRF_FLUX_UPWIND or DRF_FLUX_LF_ADV).
+ * UL, UR, rhs_l, rhs_r are [nfaces][(N+1)^2], normals [nfaces][3].  As in the
+ * reference, the face rule has N LGL points per direction (FaceQuadrature
+ * with `order` points, src/dg/quadrature_3d.cpp:13-31): face values
+ * 0..N^2-1 enter with index i + N j; the other entries of rhs_l/rhs_r are 0. */
+drf_status drf_conforming_flux_batch(int order, int kind, const double *params, int64_t nfaces,
+                                     const void *UL, const void *UR, const void *normals,
+                                     void *rhs_l, void *rhs_r, void *stream);
+
+/* MortarSpace (include/dg/mortar.hpp:32-136): projection and lifting
+ * operators of one connection type (DRF_CONN_SAME_LEVEL .. DRF_CONN_FINE_2X2)
+ * built on the host at creation and kept in device memory (device < 0: host
+ * copies only, for drf_mortar_operators; the flux batch then fails with
+ * DRF_ERR_STATE). */
+typedef struct drf_mortar drf_mortar;
+drf_status drf_mortar_create(int order, int type, int device, drf_mortar **out);
+void drf_mortar_destroy(drf_mortar *m);
+int drf_mortar_num_fine(const drf_mortar *m);
+int drf_mortar_num_dofs(const drf_mortar *m);
+/* host copies of the operators, each (N+1)^2 x (N+1)^2 row-major (Pf, Lf:
+ * num_fine of them); any output may be NULL */
+drf_status drf_mortar_operators(const drf_mortar *m, double *M, double *Minv, double *Pc,
+                                double *Lc, double *Pf, double *Lf);
+/* MortarSpace::compute_mortar_flux (src/dg/mortar.cpp:212-258) for a batch
+ * of interfaces with a scalar flux.  Uc, rhs_c [n][(N+1)^2]; Uf, rhs_f
+ * [n][num_fine][(N+1)^2]; normals [n][3]. */
+drf_status drf_mortar_flux_batch(drf_mortar *m, int kind, const double *params,
+                                 int64_t n_interfaces, const void *Uc, const void *Uf,
+                                 const void *normals, void *rhs_c, void *rhs_f, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */
 /* ------------------------------------------------------------------ */
 /* Algorithmic HBM bytes of one drf_rhs_dev launch at the current mesh. */
