@@ -110,12 +110,59 @@ def copy_probe(steps, warmup):
     }), flush=True)
 
 
+def measure_extras(args, case, s64, U, r64, stream):
+    """Mixed-precision RHS (with its error against FP64) and RK4 steps."""
+    import torch
+
+    from drifter_amd import _lib
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    E, nd = case.E, case.nd
+    dof = E * nd
+    # mixed precision (FP32 volume arithmetic and storage)
+    s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
+    s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    U32 = U.to(torch.float32)
+    R32 = torch.empty_like(U32)
+    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
+                         args.steps, args.warmup, stream)
+    err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
+    log(f"mixed RHS: {ms32 * 1e3:.1f} us, max rel L2 vs FP64 {max(err32):.2e}")
+
+    # RK4 steps: fused epilogue vs the reference's un-fused sequence
+    dt = 1e-3
+    Uw = U.clone()
+    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
+                                                    True, stream), 10, 2, stream)
+    Uw = U.clone()
+    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
+                                                    False, stream), 10, 2, stream)
+    Uw32 = U32.clone()
+    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_LINES,
+                                                    True, stream), 10, 2, stream)
+    log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
+        f"{ms_rk_m:.3f} ms")
+
+    return {
+        "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
+        "mixed_ms": round(ms32, 6),
+        "mixed_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
+        "mixed_hbm_gb_s": round(s32.rhs_bytes() / (ms32 * 1e-3) / 1e9, 1),
+        "rk4_step_ms_fp64_fused": round(ms_rk_f, 4),
+        "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
+        "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
+        "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rhs-only", action="store_true",
+                    help="time the FP64 RHS only (no mixed-precision or RK extras)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time plain copies of known size instead (HBM counter calibration "
                          "and copy ceiling)")
@@ -155,29 +202,9 @@ def main():
     r64 = R.cpu().numpy()
     log(f"FP64 RHS: {ms * 1e3:.1f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s")
 
-    # mixed precision (FP32 volume arithmetic and storage)
-    s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
-    s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U32 = U.to(torch.float32)
-    R32 = torch.empty_like(U32)
-    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
-                         args.steps, args.warmup, stream)
-    err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
-    log(f"mixed RHS: {ms32 * 1e3:.1f} us, max rel L2 vs FP64 {max(err32):.2e}")
-
-    # RK4 steps: fused epilogue vs the reference's un-fused sequence
-    dt = 1e-3
-    Uw = U.clone()
-    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
-                                                    True, stream), 10, 2, stream)
-    Uw = U.clone()
-    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
-                                                    False, stream), 10, 2, stream)
-    Uw32 = U32.clone()
-    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_LINES,
-                                                    True, stream), 10, 2, stream)
-    log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
-        f"{ms_rk_m:.3f} ms")
+    extras = None
+    if not args.rhs_only:
+        extras = measure_extras(args, case, s64, U, r64, stream)
 
     cpu = None if args.no_cpu else cpu_baseline(case)
     if cpu:
@@ -218,16 +245,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per,
         },
         "cpu_baseline": cpu,
-        "extras": {
-            "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
-            "mixed_ms": round(ms32, 6),
-            "mixed_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
-            "mixed_hbm_gb_s": round(s32.rhs_bytes() / (ms32 * 1e-3) / 1e9, 1),
-            "rk4_step_ms_fp64_fused": round(ms_rk_f, 4),
-            "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
-            "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
-            "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
-        },
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
 

@@ -73,6 +73,13 @@ SIGNATURES = {
     "drf_mortar_num_dofs": (_I, [_P]),
     "drf_mortar_operators": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "drf_mortar_flux_batch": (_I, [_P, _I, _PD, _L, _P, _P, _P, _P, _P, _P]),
+    "drf_estimator_create": (_I, [_I, _D, _D, _D, _I, _I, _I, C.POINTER(_P)]),
+    "drf_estimator_destroy": (None, [_P]),
+    "drf_element_char_length": (_I, [_L, _PD, _PD]),
+    "drf_estimate_dev": (_I, [_P, _L, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "drf_modal_matrices": (_I, [_I, _PD, _PD]),
+    "drf_modal_coeffs_dev": (_I, [_I, _L, _P, _L, _P, _P]),
+    "drf_modal_features_bf16_dev": (_I, [_I, _L, _P, _L, _D, _P, _L, _I, _P]),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

@@ -127,6 +127,9 @@ def bathymetry(kind, x, y, Lx, Ly):
         return 50.0 - 30.0 * np.exp(-r2 / (2.0 * 2000.0 ** 2))
     if kind == "shelf":  # tanh shelf from 20 m to 200 m
         return 110.0 + 90.0 * np.tanh((x - 0.5 * Lx) / (Lx / 20.0))
+    if kind == "coast":  # 20 m at the sinusoidal coastline, 200 m offshore
+        yc = 0.8 * Ly + 0.05 * Ly * np.sin(2.0 * np.pi * x / (0.25 * Lx))
+        return 20.0 + 180.0 * np.tanh(np.maximum(yc - y, 0.0) / (0.2 * Ly))
     if kind == "flat":
         return np.full_like(x, 50.0)
     raise ValueError(kind)
@@ -165,6 +168,11 @@ class Case:
 
 def make_case(nx, ny, nz, order, L=10000.0, bathy="seamount", seed_base=20261202):
     bounds = uniform_mesh(nx, ny, nz, 0.0, L, 0.0, L, -1.0, 0.0)
+    return make_case_on(bounds, order, L, bathy, seed_base)
+
+
+def make_case_on(bounds, order, L, bathy="seamount", seed_base=20261202):
+    """Seeded state on any element list (bounds (E, 6), domain [0, L]^2 x [-1, 0])."""
     nodes, _, _ = lgl_nodes(order)
     x, y, s = node_coords(bounds, nodes)
     h = bathymetry(bathy, x, y, L, L)
@@ -181,7 +189,7 @@ def make_case(nx, ny, nz, order, L=10000.0, bathy="seamount", seed_base=20261202
     U[:, 2] = eta
     U[:, 3] = H * T
     U[:, 4] = H * S
-    return Case(order, bounds, np.ascontiguousarray(h.reshape(-1)),
+    return Case(order, np.ascontiguousarray(bounds), np.ascontiguousarray(h.reshape(-1)),
                 np.ascontiguousarray(y.reshape(-1)), np.ascontiguousarray(U.reshape(-1)), L, L)
 
 
@@ -194,6 +202,18 @@ def config1():
 def config2():
     """64x64x8 elements, N=4, 100 km square, tanh shelf 20 -> 200 m."""
     return make_case(64, 64, 8, 4, 100000.0, "shelf")
+
+
+def config3(band=7000.0):
+    """Coastline-refined octree: base 32x32x4, N=3, 100 km square, three
+    rounds of XY refinement (with 2:1 balance) of the leaves within `band`
+    of y = 0.8 L + 0.05 L sin(8 pi x / L); 39,280 elements at the default.
+    Returns (case, mesh)."""
+    from .mesh import coastline_mesh
+
+    m = coastline_mesh(band=band)
+    bounds, _, _ = m.elements()
+    return make_case_on(bounds, 3, 100000.0, "coast"), m
 
 
 def to_lines(U, E, n):

@@ -256,6 +256,58 @@ drf_status drf_mortar_flux_batch(drf_mortar *m, int kind, const double *params,
                                  const void *normals, void *rhs_c, void *rhs_f, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Error estimator and element marking (GradientErrorEstimator,        */
+/* include/amr/refinement.hpp; src/amr/refinement.cpp:39-161,488-520)  */
+/* ------------------------------------------------------------------ */
+
+typedef struct drf_estimator drf_estimator;
+
+/* GradientErrorEstimator(basis, params) (refinement.cpp:39-41) with the
+ * RefinementParams fields it and mark_elements read (refinement.hpp:63-71:
+ * defaults 0.1, 0.01, 0.5, 8, 0). */
+drf_status drf_estimator_create(int order, double refine_threshold, double coarsen_threshold,
+                                double smoothness_factor, int max_level, int min_level,
+                                int device, drf_estimator **out);
+void drf_estimator_destroy(drf_estimator *est);
+
+/* Characteristic length cbrt(dx dy dz) of each element from its bounds
+ * (E*6, as drf_mesh_elements writes them), estimate_element
+ * (refinement.cpp:79-82).  Host call. */
+drf_status drf_element_char_length(int64_t n_elem, const double *bounds, double *h_char);
+
+/* GradientErrorEstimator::estimate (refinement.cpp:44-68) on one nodal field
+ * (element e at field + e*elem_stride, device), then, when levels (E*3
+ * int32, device) is not NULL, the marking of mark_elements (:488-517):
+ * action 0 None / 1 Refine / 2 Coarsen and the refine mask (XYZ when the
+ * suggestion is NONE).  Outputs (device): value[E] double (normalised by the
+ * maximum when it exceeds 1e-14), mask[E], refine[E], coarsen[E] uint8,
+ * action[E], mask_out[E] uint8.  Calls on one estimator are stream-ordered
+ * (they share its partial-max buffer). */
+drf_status drf_estimate_dev(drf_estimator *est, int64_t n_elem, const void *field,
+                            int64_t elem_stride, const void *h_char, const void *levels,
+                            void *value, void *mask, void *refine, void *coarsen, void *action,
+                            void *mask_out, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Modal coefficients (no counterpart in the reference: the modal      */
+/* Vandermonde of include/dg/basis.hpp:40-42 is declared only)         */
+/* ------------------------------------------------------------------ */
+
+/* V[i][p] = sqrt((2p+1)/2) P_p(x_i) on the LGL nodes and V^-1, n x n
+ * row-major (host). */
+drf_status drf_modal_matrices(int order, double *V, double *Vinv);
+/* coef[e][p + n(q + n r)] = sum Vinv[p][i] Vinv[q][j] Vinv[r][k] u[e][i + n(j + n k)]
+ * (device; u element e at u + e*elem_stride, coef dense [E][(N+1)^3]) */
+drf_status drf_modal_coeffs_dev(int order, int64_t n_elem, const void *u, int64_t elem_stride,
+                                void *coef, void *stream);
+/* Same coefficients times `scale`, rounded to BF16 (nearest even) into
+ * features[e * row_stride + col_offset + m] (uint16 BF16 bits): the input
+ * rows of the refinement MLP. */
+drf_status drf_modal_features_bf16_dev(int order, int64_t n_elem, const void *u,
+                                       int64_t elem_stride, double scale, void *features,
+                                       int64_t row_stride, int col_offset, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */
 /* ------------------------------------------------------------------ */
 /* Algorithmic HBM bytes of one drf_rhs_dev launch at the current mesh. */

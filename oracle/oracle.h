@@ -132,6 +132,28 @@ int orc_mortar_operators(int order, int type, double *M, double *Minv, double *P
 int orc_mortar_flux(int order, int type, int kind, const double *p, const double *Uc,
                     const double *Uf, const double *n, double *rhs_c, double *rhs_f);
 
+/* ---- A19: GradientErrorEstimator + mark_elements ---- */
+/* GradientErrorEstimator::estimate for E elements of one nodal field
+ * (element e at field + e*elem_stride), element bounds E*6 (xmin xmax ymin
+ * ymax zmin zmax).  Outputs value[E], mask[E] (RefineMask bits), refine[E],
+ * coarsen[E]. */
+void orc_estimate(int order, int64_t E, const double *field, int64_t elem_stride,
+                  const double *bounds, double refine_threshold, double coarsen_threshold,
+                  double smoothness_factor, double *value, uint8_t *mask, uint8_t *refine,
+                  uint8_t *coarsen);
+/* AdaptiveMeshRefinement::mark_elements after the estimate: action[E]
+ * (0 None, 1 Refine, 2 Coarsen) and mask_out[E]; levels E*3. */
+void orc_mark_elements(int64_t E, const uint8_t *refine, const uint8_t *coarsen,
+                       const uint8_t *mask, const int *levels, int max_level, int min_level,
+                       uint8_t *action, uint8_t *mask_out);
+
+/* ---- A20: nodal -> orthonormal Legendre modal coefficients ---- */
+/* V[i][p] = sqrt((2p+1)/2) P_p(x_i) on the LGL nodes (n x n row-major) and
+ * its inverse */
+int orc_modal_matrices(int order, double *V, double *Vinv);
+/* coef[e][p + n(q + n r)] = sum Vinv[p][i] Vinv[q][j] Vinv[r][k] u[e][i + n(j + n k)] */
+int orc_modal_coeffs(int order, int64_t E, const double *u, int64_t elem_stride, double *coef);
+
 #ifdef __cplusplus
 }
 #endif

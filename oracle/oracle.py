@@ -66,6 +66,12 @@ def load():
         "orc_conforming_flux": (C.c_int, [C.c_int, C.c_int, _PD, _PD, _PD, _PD, _PD, _PD]),
         "orc_mortar_operators": (C.c_int, [C.c_int, C.c_int] + [_PD] * 6),
         "orc_mortar_flux": (C.c_int, [C.c_int, C.c_int, C.c_int, _PD, _PD, _PD, _PD, _PD, _PD]),
+        "orc_estimate": (None, [C.c_int, C.c_int64, _PD, C.c_int64, _PD, C.c_double, C.c_double,
+                                C.c_double, _PD, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_mark_elements": (None, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+        "orc_modal_matrices": (C.c_int, [C.c_int, _PD, _PD]),
+        "orc_modal_coeffs": (C.c_int, [C.c_int, C.c_int64, _PD, C.c_int64, _PD]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -273,3 +279,43 @@ def mortar_flux(order, ctype, kind, p, Uc, Uf, n):
     nfine = load().orc_mortar_flux(order, ctype, kind, _p(_v(p)), _p(_v(Uc)), _p(_v(Uf).reshape(-1)),
                                    _p(_v(n)), _p(rc), _p(rf))
     return rc, rf[:nfine * nm].reshape(nfine, nm)
+
+
+def estimate(order, field, bounds, refine_threshold=0.1, coarsen_threshold=0.01,
+             smoothness_factor=0.5):
+    """GradientErrorEstimator::estimate; field (E, ndof), bounds (E, 6)."""
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    bounds = np.ascontiguousarray(bounds, dtype=np.float64)
+    E = field.shape[0]
+    value = np.zeros(E)
+    mask, refine, coarsen = (np.zeros(E, dtype=np.uint8) for _ in range(3))
+    load().orc_estimate(order, E, _p(field), field.shape[1], _p(bounds), refine_threshold,
+                        coarsen_threshold, smoothness_factor, _p(value), mask.ctypes.data,
+                        refine.ctypes.data, coarsen.ctypes.data)
+    return value, mask, refine, coarsen
+
+
+def mark_elements(refine, coarsen, mask, levels, max_level=8, min_level=0):
+    E = len(refine)
+    refine, coarsen, mask = (np.ascontiguousarray(a, dtype=np.uint8) for a in (refine, coarsen, mask))
+    levels = np.ascontiguousarray(levels, dtype=np.int32)
+    action, mout = np.zeros(E, dtype=np.uint8), np.zeros(E, dtype=np.uint8)
+    load().orc_mark_elements(E, refine.ctypes.data, coarsen.ctypes.data, mask.ctypes.data,
+                             levels.ctypes.data, max_level, min_level, action.ctypes.data,
+                             mout.ctypes.data)
+    return action, mout
+
+
+def modal_matrices(order):
+    n = order + 1
+    V, Vi = np.zeros((n, n)), np.zeros((n, n))
+    assert load().orc_modal_matrices(order, _p(V), _p(Vi)) == 0
+    return V, Vi
+
+
+def modal_coeffs(order, u):
+    """u (E, ndof) nodal -> (E, ndof) orthonormal Legendre coefficients"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    assert load().orc_modal_coeffs(order, u.shape[0], _p(u), u.shape[1], _p(out)) == 0
+    return out
