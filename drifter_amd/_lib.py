@@ -76,10 +76,19 @@ SIGNATURES = {
     "drf_estimator_create": (_I, [_I, _D, _D, _D, _I, _I, _I, C.POINTER(_P)]),
     "drf_estimator_destroy": (None, [_P]),
     "drf_element_char_length": (_I, [_L, _PD, _PD]),
-    "drf_estimate_dev": (_I, [_P, _L, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "drf_estimate_dev": (_I, [_P, _L, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "drf_modal_matrices": (_I, [_I, _PD, _PD]),
     "drf_modal_coeffs_dev": (_I, [_I, _L, _P, _L, _P, _P]),
-    "drf_modal_features_bf16_dev": (_I, [_I, _L, _P, _L, _D, _P, _L, _I, _P]),
+    "drf_modal_features_bf16_dev": (_I, [_I, _L, _P, _L, _D, _P, _P, _L, _I, _P]),
+    "drf_mlp_create": (_I, [C.c_uint64, _I, C.POINTER(_P)]),
+    "drf_mlp_destroy": (None, [_P]),
+    "drf_mlp_set_params": (_I, [_P, _P]),
+    "drf_mlp_get_params": (_I, [_P, _P]),
+    "drf_mlp_forward_dev": (_I, [_P, _L, _P, _P, _P]),
+    "drf_mlp_classify_dev": (_I, [_L, _P, _D, _D, _P, _P]),
+    "drf_mlp_loss_grad_dev": (_I, [_P, _L, _P, _P, _P, _P, _P]),
+    "drf_mlp_train_step_dev": (_I, [_P, _L, _P, _P, _D, _P, _P]),
+    "drf_sigma_metrics_dev": (_I, [_L] + [_P] * 14),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

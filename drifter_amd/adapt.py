@@ -46,14 +46,16 @@ class GradientErrorEstimator:
     def estimate(self, field, h_char, levels=None, elem_stride=None, stream=None):
         """field: device float64 with element e at e*elem_stride; h_char (E,)
         device float64; levels (E, 3) device int32 or None.  Returns a dict of
-        device tensors value, mask, refine, coarsen (+ action, mask_out)."""
+        device tensors value, mask, refine, coarsen, max_value (the
+        un-normalised maximum, one double) (+ action, mask_out)."""
         import torch
 
         nd = (self.order + 1) ** 3
         es = nd if elem_stride is None else elem_stride
         E = h_char.numel()
         dev = h_char.device
-        out = {"value": torch.empty(E, dtype=torch.float64, device=dev)}
+        out = {"value": torch.empty(E, dtype=torch.float64, device=dev),
+               "max_value": torch.empty(1, dtype=torch.float64, device=dev)}
         for k in ("mask", "refine", "coarsen"):
             out[k] = torch.empty(E, dtype=torch.uint8, device=dev)
         if levels is not None:
@@ -66,7 +68,7 @@ class GradientErrorEstimator:
         check(self._l.drf_estimate_dev(
             self._e, E, _ptr(field), es, _ptr(h_char), _ptr(levels), _ptr(out["value"]),
             _ptr(out["mask"]), _ptr(out["refine"]), _ptr(out["coarsen"]), _ptr(out.get("action")),
-            _ptr(out.get("mask_out")), _sptr(stream)))
+            _ptr(out.get("mask_out")), _ptr(out["max_value"]), _sptr(stream)))
         return out
 
 
@@ -89,16 +91,19 @@ def modal_coeffs(order, u, n_elem, elem_stride=None, stream=None):
     return coef
 
 
-def modal_features_bf16(order, u, n_elem, features, col_offset, scale=1.0, elem_stride=None,
-                        stream=None):
-    """Write scaled modal coefficients as BF16 into features (E, row) int16/bfloat16."""
+def modal_features_bf16(order, u, n_elem, features, col_offset, scale=1.0, elem_scale=None,
+                        elem_stride=None, stream=None):
+    """Write scale * elem_scale[e] * (modal coefficients, mean mode zeroed) as
+    BF16 into features (E, row) 2-byte device tensor at col_offset."""
     nd = (order + 1) ** 3
     es = nd if elem_stride is None else elem_stride
     if features.element_size() != 2 or features.shape[0] < n_elem:
         raise ValueError("features must be a 2-byte (E, row) device tensor")
     if n_elem and u.numel() < (n_elem - 1) * es + nd:
         raise ValueError("u is too small for n_elem and elem_stride")
+    if elem_scale is not None and elem_scale.numel() < n_elem:
+        raise ValueError("elem_scale needs one value per element")
     check(_lib.load().drf_modal_features_bf16_dev(order, n_elem, _ptr(u), es, scale,
-                                                  _ptr(features), features.shape[1], col_offset,
-                                                  _sptr(stream)))
+                                                  _ptr(elem_scale), _ptr(features),
+                                                  features.shape[1], col_offset, _sptr(stream)))
     return features

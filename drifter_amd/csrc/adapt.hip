@@ -123,7 +123,8 @@ __global__ void __launch_bounds__(64 * kWaves)
 __global__ void __launch_bounds__(256)
     normalize_mark_kernel(int64_t E, const double *__restrict__ partial, int npart,
                           EstimatorParams p, const int32_t *__restrict__ levels,
-                          double *__restrict__ value, const uint8_t *__restrict__ mask,
+                          double *__restrict__ max_out, double *__restrict__ value,
+                          const uint8_t *__restrict__ mask,
                           uint8_t *__restrict__ refine, uint8_t *__restrict__ coarsen,
                           uint8_t *__restrict__ action, uint8_t *__restrict__ mask_out)
 {
@@ -136,6 +137,8 @@ __global__ void __launch_bounds__(256)
         red[threadIdx.x >> 6] = m;
     __syncthreads();
     const double mx = dmax(dmax(red[0], red[1]), dmax(red[2], red[3]));
+    if (max_out && blockIdx.x == 0 && threadIdx.x == 0)
+        *max_out = mx;
     const bool norm = mx > 1e-14;
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * 256) {
@@ -181,7 +184,8 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 template <class OUT>
 __global__ void __launch_bounds__(64 * kWaves)
     modal_kernel(int n, int64_t E, const double *__restrict__ u, int64_t es, ModalOps ops,
-                 OUT *__restrict__ out, int64_t out_stride, int out_offset, double scale)
+                 OUT *__restrict__ out, int64_t out_stride, int out_offset, double scale,
+                 const double *__restrict__ elem_scale)
 {
     extern __shared__ double sm[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -215,6 +219,7 @@ __global__ void __launch_bounds__(64 * kWaves)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         OUT *dst = out + e * out_stride + out_offset;
+        const double sc = elem_scale ? scale * elem_scale[e] : scale;
         for (int d = lane; d < nd; d += 64) { // coef[p + n(q + n r)] = sum_k Vi[r][k] a[p + n(q + n k)]
             const int pq = d % n2, r = d / n2;
             double s = 0.0;
@@ -222,8 +227,8 @@ __global__ void __launch_bounds__(64 * kWaves)
                 s += ops.Vinv[r * kMaxN + k] * a[k * n2 + pq];
             if constexpr (sizeof(OUT) == 8)
                 dst[d] = s;
-            else
-                dst[d] = to_bf16((float)(s * scale));
+            else // features: the element mean (mode 0) carries no variation
+                dst[d] = (d == 0) ? (uint16_t)0 : to_bf16((float)(s * sc));
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -241,7 +246,7 @@ unsigned grid_for(int64_t E, int64_t cap)
 
 hipError_t launch_estimate(int n, int64_t E, const double *field, int64_t es, const double *hchar,
                            const int32_t *levels, const EstimatorParams &p, double *partial,
-                           double *value, uint8_t *mask, uint8_t *refine, uint8_t *coarsen,
+                           double *max_out, double *value, uint8_t *mask, uint8_t *refine, uint8_t *coarsen,
                            uint8_t *action, uint8_t *mask_out, hipStream_t s)
 {
     if (E <= 0)
@@ -256,13 +261,13 @@ hipError_t launch_estimate(int n, int64_t E, const double *field, int64_t es, co
     const int64_t g2n = (E + 255) / 256;
     const unsigned g2 = (unsigned)(g2n < 1024 ? g2n : 1024);
     hipLaunchKernelGGL(normalize_mark_kernel, dim3(g2), dim3(256), 0, s, E, partial, (int)g1, p,
-                       levels, value, mask, refine, coarsen, action, mask_out);
+                       levels, max_out, value, mask, refine, coarsen, action, mask_out);
     return hipGetLastError();
 }
 
 hipError_t launch_modal(int n, int64_t E, const double *u, int64_t es, const ModalOps &ops,
                         double *coef, uint16_t *bf16, int64_t out_stride, int out_offset,
-                        double scale, hipStream_t s)
+                        double scale, const double *elem_scale, hipStream_t s)
 {
     if (E <= 0)
         return hipSuccess;
@@ -270,10 +275,10 @@ hipError_t launch_modal(int n, int64_t E, const double *u, int64_t es, const Mod
     const size_t lds = sizeof(double) * (size_t)kWaves * 2 * n * n * n;
     if (coef)
         hipLaunchKernelGGL(modal_kernel<double>, dim3(g), dim3(64 * kWaves), lds, s, n, E, u, es,
-                           ops, coef, out_stride, out_offset, scale);
+                           ops, coef, out_stride, out_offset, scale, elem_scale);
     else
         hipLaunchKernelGGL(modal_kernel<uint16_t>, dim3(g), dim3(64 * kWaves), lds, s, n, E, u, es,
-                           ops, bf16, out_stride, out_offset, scale);
+                           ops, bf16, out_stride, out_offset, scale, elem_scale);
     return hipGetLastError();
 }
 

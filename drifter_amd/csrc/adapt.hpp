@@ -31,10 +31,10 @@ int modal_matrices(int n, double *V, double *Vinv);
 
 hipError_t launch_estimate(int n, int64_t E, const double *field, int64_t es, const double *hchar,
                            const int32_t *levels, const EstimatorParams &p, double *partial,
-                           double *value, uint8_t *mask, uint8_t *refine, uint8_t *coarsen,
+                           double *max_out, double *value, uint8_t *mask, uint8_t *refine, uint8_t *coarsen,
                            uint8_t *action, uint8_t *mask_out, hipStream_t s);
 hipError_t launch_modal(int n, int64_t E, const double *u, int64_t es, const ModalOps &ops,
                         double *coef, uint16_t *bf16, int64_t out_stride, int out_offset,
-                        double scale, hipStream_t s);
+                        double scale, const double *elem_scale, hipStream_t s);
 
 } // namespace drf

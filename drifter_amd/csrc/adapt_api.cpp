@@ -143,7 +143,7 @@ void drf_estimator_destroy(drf_estimator *est)
 drf_status drf_estimate_dev(drf_estimator *est, int64_t n_elem, const void *field,
                             int64_t elem_stride, const void *h_char, const void *levels,
                             void *value, void *mask, void *refine, void *coarsen, void *action,
-                            void *mask_out, void *stream)
+                            void *mask_out, void *max_value, void *stream)
 {
     DRF_REQUIRE(est, DRF_ERR_ARG, "drf_estimate_dev: null estimator");
     const int n = est->order + 1;
@@ -158,7 +158,8 @@ drf_status drf_estimate_dev(drf_estimator *est, int64_t n_elem, const void *fiel
     DRF_HIP_CHECK(hipSetDevice(est->device));
     DRF_HIP_CHECK(launch_estimate(n, n_elem, (const double *)field, elem_stride,
                                   (const double *)h_char, (const int32_t *)levels, est->p,
-                                  est->d_partial, (double *)value, (uint8_t *)mask,
+                                  est->d_partial, (double *)max_value, (double *)value,
+                                  (uint8_t *)mask,
                                   (uint8_t *)refine, (uint8_t *)coarsen, (uint8_t *)action,
                                   (uint8_t *)mask_out, (hipStream_t)stream));
     return DRF_OK;
@@ -188,13 +189,14 @@ drf_status drf_modal_coeffs_dev(int order, int64_t n_elem, const void *u, int64_
     const ModalOps ops = modal_ops(n, ok);
     DRF_REQUIRE(ok, DRF_ERR_ARG, "drf_modal_coeffs_dev: singular Vandermonde");
     DRF_HIP_CHECK(launch_modal(n, n_elem, (const double *)u, elem_stride, ops, (double *)coef,
-                               nullptr, nd, 0, 1.0, (hipStream_t)stream));
+                               nullptr, nd, 0, 1.0, nullptr, (hipStream_t)stream));
     return DRF_OK;
 }
 
 drf_status drf_modal_features_bf16_dev(int order, int64_t n_elem, const void *u,
-                                       int64_t elem_stride, double scale, void *features,
-                                       int64_t row_stride, int col_offset, void *stream)
+                                       int64_t elem_stride, double scale, const void *elem_scale,
+                                       void *features, int64_t row_stride, int col_offset,
+                                       void *stream)
 {
     DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_UNSUPPORTED,
                 "drf_modal_features_bf16_dev: order out of range");
@@ -210,7 +212,7 @@ drf_status drf_modal_features_bf16_dev(int order, int64_t n_elem, const void *u,
     DRF_REQUIRE(ok, DRF_ERR_ARG, "drf_modal_features_bf16_dev: singular Vandermonde");
     DRF_HIP_CHECK(launch_modal(n, n_elem, (const double *)u, elem_stride, ops, nullptr,
                                (uint16_t *)features, row_stride, col_offset, scale,
-                               (hipStream_t)stream));
+                               (const double *)elem_scale, (hipStream_t)stream));
     return DRF_OK;
 }
 

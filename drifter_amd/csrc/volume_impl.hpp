@@ -136,6 +136,8 @@ __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS>
     }
 }
 
+constexpr int kXcds = 8; // MI355X: 8 XCDs of 32 CUs
+
 template <int N, class TS, class TC, int EPI>
 __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     volume_kernel(const VolGeom<TS> geo, const KOps<N + 1, TC> ops, const Phys ph,
@@ -160,15 +162,24 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
     const TC *__restrict__ D = ops.D;
     const TC *__restrict__ D2 = ops.D2;
 
+    // XCD-aware batch order.  Workgroups are dealt round-robin to the 8 XCDs
+    // (blockIdx % 8), each with its own L2.  Renumbering them so that
+    // consecutive logical workgroups sit on one XCD puts neighbouring
+    // batches -- whose line spans share a 128-byte cache line, the spans not
+    // being line-aligned -- behind the same L2, while the grid as a whole
+    // still sweeps memory in one contiguous window per round.
+    const int64_t G = gridDim.x;
+    const int64_t lw = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
+                                        : (int64_t)blockIdx.x;
+    int64_t b = lw;
     Lines<n, TC> nxt;
-    int64_t b = blockIdx.x;
     if (PREFETCH) {
         const int64_t e0 = b * EPB + le;
         if (in_tile && b < nbatch && e0 < geo.E)
             load_lines<n, TS, TC>(nxt, U, geo.h, e0, L);
     }
 
-    for (; b < nbatch; b += gridDim.x) {
+    for (; b < nbatch; b += G) {
         const int64_t e = b * EPB + le;
         const bool act = in_tile && e < geo.E;
         Lines<n, TC> cur;
@@ -281,7 +292,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
         TC o1[5][n], o2[5][n];
         if (PREFETCH) {
             // Next batch of this workgroup, in flight during phases 2-4.
-            const int64_t bn = b + gridDim.x;
+            const int64_t bn = b + G;
             const int64_t en = bn * EPB + le;
             if (in_tile && bn < nbatch && en < geo.E)
                 load_lines<n, TS, TC>(nxt, U, geo.h, en, L);

@@ -281,12 +281,13 @@ drf_status drf_element_char_length(int64_t n_elem, const double *bounds, double 
  * action 0 None / 1 Refine / 2 Coarsen and the refine mask (XYZ when the
  * suggestion is NONE).  Outputs (device): value[E] double (normalised by the
  * maximum when it exceeds 1e-14), mask[E], refine[E], coarsen[E] uint8,
- * action[E], mask_out[E] uint8.  Calls on one estimator are stream-ordered
- * (they share its partial-max buffer). */
+ * action[E], mask_out[E] uint8, and (when max_value is not NULL) the
+ * un-normalised maximum as one double.  Calls on one estimator are
+ * stream-ordered (they share its partial-max buffer). */
 drf_status drf_estimate_dev(drf_estimator *est, int64_t n_elem, const void *field,
                             int64_t elem_stride, const void *h_char, const void *levels,
                             void *value, void *mask, void *refine, void *coarsen, void *action,
-                            void *mask_out, void *stream);
+                            void *mask_out, void *max_value, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Modal coefficients (no counterpart in the reference: the modal      */
@@ -300,12 +301,64 @@ drf_status drf_modal_matrices(int order, double *V, double *Vinv);
  * (device; u element e at u + e*elem_stride, coef dense [E][(N+1)^3]) */
 drf_status drf_modal_coeffs_dev(int order, int64_t n_elem, const void *u, int64_t elem_stride,
                                 void *coef, void *stream);
-/* Same coefficients times `scale`, rounded to BF16 (nearest even) into
+/* Same coefficients times scale * elem_scale[e] (elem_scale: device double
+ * per element, or NULL for 1), rounded to BF16 (nearest even) into
  * features[e * row_stride + col_offset + m] (uint16 BF16 bits): the input
- * rows of the refinement MLP. */
+ * rows of the refinement MLP.  Mode m = 0, the element mean, is written as
+ * zero (it carries no variation). */
 drf_status drf_modal_features_bf16_dev(int order, int64_t n_elem, const void *u,
-                                       int64_t elem_stride, double scale, void *features,
-                                       int64_t row_stride, int col_offset, void *stream);
+                                       int64_t elem_stride, double scale, const void *elem_scale,
+                                       void *features, int64_t row_stride, int col_offset,
+                                       void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Refinement-indicator MLP (no counterpart in the reference): 128 -> 64 */
+/* -> 64 -> 1, GELU (tanh form), BF16 GEMM operands with FP32           */
+/* accumulation, MSE loss, Adam                                        */
+/* ------------------------------------------------------------------ */
+
+#define DRF_MLP_IN 128
+#define DRF_MLP_HIDDEN 64
+#define DRF_MLP_NPARAM 12481 /* W1[64][128] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 */
+
+typedef struct drf_mlp drf_mlp;
+/* Glorot-uniform weights from a SplitMix64 stream seeded with `seed`, zero
+ * biases; FP32 master weights and Adam state on `device`. */
+drf_status drf_mlp_create(uint64_t seed, int device, drf_mlp **out);
+void drf_mlp_destroy(drf_mlp *m);
+/* host FP32 parameter vectors (DRF_MLP_NPARAM); set also clears Adam */
+drf_status drf_mlp_set_params(drf_mlp *m, const float *params);
+drf_status drf_mlp_get_params(const drf_mlp *m, float *params);
+/* pred[n] (FP32) for X[n][128] BF16 (uint16 bits, 16-byte aligned) */
+drf_status drf_mlp_forward_dev(drf_mlp *m, int64_t n, const void *X, void *pred, void *stream);
+/* action[n] (uint8): 1 (refine) if pred > refine_level, 2 (coarsen) if
+ * pred < coarsen_level, else 0 -- RefinementAction numbering */
+drf_status drf_mlp_classify_dev(int64_t n, const void *pred, double refine_level,
+                                double coarsen_level, void *action, void *stream);
+/* gradient (FP32, DRF_MLP_NPARAM, device) and loss (device float) of the
+ * mean squared error over the n rows of X against y (FP32) */
+drf_status drf_mlp_loss_grad_dev(drf_mlp *m, int64_t n, const void *X, const void *y, void *grad,
+                                 void *loss, void *stream);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on that loss; loss may be NULL.
+ * Steps on one model are stream-ordered (they share its scratch buffers). */
+drf_status drf_mlp_train_step_dev(drf_mlp *m, int64_t n, const void *X, const void *y, double lr,
+                                  void *loss, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Sigma-coordinate metrics (SigmaMetrics::update,                     */
+/* include/mesh/sigma_coordinate.hpp:340-354); opt-in, FP64            */
+/* ------------------------------------------------------------------ */
+
+/* Per node (device arrays of npts): H = eta + h, Hinv (0 when H <= 1e-10),
+ * dsigma/dx = -(deta/dx + sigma (deta/dx + dh/dx)) / H, dsigma/dy likewise,
+ * dsigma/dt = -(deta/dt (1 + sigma)) / H (all three 0 when H < 1e-10).
+ * deta_dt may be NULL (update_steady, :357-361).  The right-hand side does
+ * not use these terms, as in the reference. */
+drf_status drf_sigma_metrics_dev(int64_t npts, const void *sigma, const void *eta,
+                                 const void *h, const void *deta_dx, const void *deta_dy,
+                                 const void *dh_dx, const void *dh_dy, const void *deta_dt,
+                                 void *H, void *Hinv, void *dsigma_dx, void *dsigma_dy,
+                                 void *dsigma_dt, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */

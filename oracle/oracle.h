@@ -154,6 +154,24 @@ int orc_modal_matrices(int order, double *V, double *Vinv);
 /* coef[e][p + n(q + n r)] = sum Vinv[p][i] Vinv[q][j] Vinv[r][k] u[e][i + n(j + n k)] */
 int orc_modal_coeffs(int order, int64_t E, const double *u, int64_t elem_stride, double *coef);
 
+/* ---- A13: sigma-coordinate metrics (SigmaMetrics::update) ---- */
+/* deta_dt may be NULL (update_steady) */
+void orc_sigma_metrics(int64_t npts, const double *sigma, const double *eta, const double *h,
+                       const double *deta_dx, const double *deta_dy, const double *dh_dx,
+                       const double *dh_dy, const double *deta_dt, double *H, double *Hinv,
+                       double *dsdx, double *dsdy, double *dsdt);
+
+/* ---- A21: refinement-indicator MLP (128 -> 64 -> 64 -> 1, GELU) ---- */
+#define ORC_MLP_IN 128
+#define ORC_MLP_HIDDEN 64
+#define ORC_MLP_NPARAM 12481
+/* BF16 round-to-nearest-even of (float)x, returned as a double */
+double orc_bf16_round(double x);
+/* mean squared error over B rows of X (B x 128) against y; pred[B] and
+ * grad[12481] (d loss / d params) optional; quantize: see mlp.c */
+double orc_mlp_loss_grad(const double *params, int64_t B, const double *X, const double *y,
+                         int quantize, double *pred, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

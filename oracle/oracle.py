@@ -72,6 +72,9 @@ def load():
                                      C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
         "orc_modal_matrices": (C.c_int, [C.c_int, _PD, _PD]),
         "orc_modal_coeffs": (C.c_int, [C.c_int, C.c_int64, _PD, C.c_int64, _PD]),
+        "orc_sigma_metrics": (None, [C.c_int64] + [_PD] * 13),
+        "orc_bf16_round": (C.c_double, [C.c_double]),
+        "orc_mlp_loss_grad": (C.c_double, [_PD, C.c_int64, _PD, _PD, C.c_int, _PD, _PD]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -319,3 +322,28 @@ def modal_coeffs(order, u):
     out = np.zeros_like(u)
     assert load().orc_modal_coeffs(order, u.shape[0], _p(u), u.shape[1], _p(out)) == 0
     return out
+
+
+def sigma_metrics(sigma, eta, h, deta_dx, deta_dy, dh_dx, dh_dy, deta_dt=None):
+    """SigmaMetrics::update -> (H, Hinv, dsigma_dx, dsigma_dy, dsigma_dt)"""
+    a = [_v(x) for x in (sigma, eta, h, deta_dx, deta_dy, dh_dx, dh_dy)]
+    n = a[0].size
+    et = None if deta_dt is None else _v(deta_dt)
+    outs = [np.zeros(n) for _ in range(5)]
+    load().orc_sigma_metrics(n, *[_p(x) for x in a], None if et is None else _p(et),
+                             *[_p(o) for o in outs])
+    return tuple(outs)
+
+
+MLP_IN, MLP_HIDDEN, MLP_NPARAM = 128, 64, 12481
+
+
+def mlp_loss_grad(params, X, y, quantize=True, want_grad=True):
+    """-> (loss, pred, grad) of the MSE loss of the indicator MLP"""
+    params, X, y = _v(params), _v(X), _v(y)
+    B = X.shape[0]
+    pred = np.zeros(B)
+    grad = np.zeros(MLP_NPARAM) if want_grad else None
+    loss = load().orc_mlp_loss_grad(_p(params), B, _p(X), _p(y), int(quantize), _p(pred),
+                                    None if grad is None else _p(grad))
+    return loss, pred, grad

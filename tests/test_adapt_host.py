@@ -33,9 +33,9 @@ def test_estimator_entry_points_reject_bad_arguments():
 
     lib = _lib.load()
     assert lib.drf_estimate_dev(None, 10, None, 64, None, None, None, None, None, None, None,
-                                None, None) == _lib.DRF_ERR_ARG
+                                None, None, None) == _lib.DRF_ERR_ARG
     assert lib.drf_modal_coeffs_dev(3, 10, None, 63, None, None) == _lib.DRF_ERR_ARG
     assert lib.drf_modal_coeffs_dev(9, 10, None, 1000, None, None) != _lib.DRF_OK
-    assert lib.drf_modal_features_bf16_dev(3, 10, None, 64, 1.0, None, 100, 40,
+    assert lib.drf_modal_features_bf16_dev(3, 10, None, 64, 1.0, None, None, 100, 40,
                                            None) == _lib.DRF_ERR_ARG  # 40 + 64 > 100
     assert lib.drf_modal_coeffs_dev(3, 0, None, 64, None, None) == _lib.DRF_OK

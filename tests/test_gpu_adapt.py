@@ -33,6 +33,12 @@ def _check(orc, order, field, bounds, levels, got, **kw):
         kw.get("smoothness_factor", 0.5)
     nd = (order + 1) ** 3
     v, m, r, c = orc.estimate(order, field[:, :nd], bounds, rt, ct, sf)
+    # un-normalised maximum: (max - min) * cbrt(volume), the same operations
+    from drifter_amd import adapt
+
+    f = field[:, :nd]
+    raw = (f.max(axis=1) - f.min(axis=1)) * adapt.element_char_length(bounds)
+    assert got["max_value"][0] == raw.max()
     np.testing.assert_array_equal(got["value"], v)
     np.testing.assert_array_equal(got["mask"], m)
     np.testing.assert_array_equal(got["refine"], r)
@@ -129,13 +135,18 @@ def test_modal_coeffs(torch, orc, order):
     coef = adapt.modal_coeffs(order, d, E, elem_stride=nd + 5).cpu().numpy()
     ref = orc.modal_coeffs(order, u[:, :nd])
     np.testing.assert_allclose(coef, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
-    # BF16 features into columns [7, 7 + nd) of a wider row
+    # BF16 features into columns [7, 7 + nd) of a wider row, per-element scale
     feat = torch.zeros((E, nd + 16), dtype=torch.int16, device="cuda")
-    adapt.modal_features_bf16(order, d, E, feat, 7, scale=0.25, elem_stride=nd + 5)
+    es = rng.uniform(0.5, 2.0, E)
+    adapt.modal_features_bf16(order, d, E, feat, 7, scale=0.25,
+                              elem_scale=torch.from_numpy(es).cuda(), elem_stride=nd + 5)
     fb = feat.cpu().numpy().view(np.uint16)
-    exp = _bf16_rne(ref * 0.25)
+    exp = _bf16_rne(ref * (0.25 * es)[:, None])
+    exp[:, 0] = 0  # mean mode zeroed
     got = fb[:, 7:7 + nd]
     assert np.all(fb[:, :7] == 0) and np.all(fb[:, 7 + nd:] == 0)
     diff = np.abs(got.astype(np.int64) - exp.astype(np.int64))
     assert diff.max() <= 1 and (diff == 0).mean() > 0.999
-    np.testing.assert_allclose(_bf16_to_f32(got), ref * 0.25, rtol=2 ** -8, atol=1e-30)
+    sref = ref * (0.25 * es)[:, None]
+    sref[:, 0] = 0.0
+    np.testing.assert_allclose(_bf16_to_f32(got), sref, rtol=2 ** -8, atol=1e-30)
