@@ -110,7 +110,7 @@ def copy_probe(steps, warmup):
     }), flush=True)
 
 
-def measure_extras(args, case, s64, U, r64, stream):
+def measure_extras(args, case, s64, U, r64, stream, tune=None):
     """Mixed-precision RHS (with its error against FP64) and RK4 steps."""
     import torch
 
@@ -124,6 +124,7 @@ def measure_extras(args, case, s64, U, r64, stream):
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
     R32 = torch.empty_like(U32)
+    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=5, stream=stream)
     ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
                          args.steps, args.warmup, stream)
     err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
@@ -143,7 +144,26 @@ def measure_extras(args, case, s64, U, r64, stream):
     log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
         f"{ms_rk_m:.3f} ms")
 
+    # mixed precision vs FP64 after 1 and 100 fused RK4 steps from the same
+    # state (dt = 1e-5: the reference-coordinate horizontal diffusion
+    # H nu_h D^2 is stiff at N = 4; the FP64 GPU path is the reference here,
+    # being within 1e-12 of the oracle per RHS)
+    steps_err = {}
+    Ua, Ub = U.clone(), U32.clone()
+    for k in range(1, 101):
+        s64.rk_step_dev(_lib.RK_RK4, 0.0, 1e-5, Ua, _lib.LAYOUT_LINES, True, stream)
+        s32.rk_step_dev(_lib.RK_RK4, 0.0, 1e-5, Ub, _lib.LAYOUT_LINES, True, stream)
+        if k in (1, 100):
+            e = rel_l2_fields(Ub.double().cpu().numpy(), Ua.cpu().numpy(), E, nd)
+            steps_err[k] = [float("%.3e" % x) for x in e]
+    log(f"mixed vs FP64 rel L2 after 1 / 100 RK4 steps: {steps_err[1]} / {steps_err[100]}")
     return {
+        "autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1], "us": round(tune[2], 2)}
+        if tune else None,
+        "autotune_mixed": {"wg_per_cu": tune32[0], "xcd_remap": tune32[1],
+                           "us": round(tune32[2], 2)},
+        "mixed_rel_l2_after_1_rk4_step": steps_err[1],
+        "mixed_rel_l2_after_100_rk4_steps": steps_err[100],
         "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
         "mixed_ms": round(ms32, 6),
         "mixed_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
@@ -152,6 +172,77 @@ def measure_extras(args, case, s64, U, r64, stream):
         "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
         "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
         "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
+    }
+
+
+def measure_coastline(args, stream):
+    """RHS throughput on the coastline-refined octree (39,280 elements, N=3)."""
+    import torch
+
+    from drifter_amd import _lib, synth
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    case, _ = synth.config3()
+    E, nd = case.E, case.nd
+    out = {}
+    for name, prec, dt in (("fp64", _lib.PREC_FP64, torch.float64),
+                           ("mixed", _lib.PREC_MIXED, torch.float32)):
+        s = PrimitiveEquationsSolver(3, prec)
+        s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+        U = torch.from_numpy(synth.to_lines(case.U, E, 4)).to("cuda").to(dt)
+        R = torch.empty_like(U)
+        s.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
+        ms = time_launches(lambda: s.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
+                           args.steps, args.warmup, stream)
+        out[f"coastline_39280_n3_{name}_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
+        out[f"coastline_39280_n3_{name}_us"] = round(ms * 1e3, 2)
+    log(f"coastline N=3 RHS: {out}")
+    return out
+
+
+def measure_indicator(stream):
+    """Learned indicator: training on 50 seeded coastline states (about 2M
+    element/label pairs), misclassification against the estimator on 5
+    unseen seeds, and inference time on 196,400 elements (5 states) beside
+    the plain estimator on the same two fields."""
+    import torch
+
+    from drifter_amd import adapt, indicator, mlp
+    from drifter_amd.mesh import coastline_mesh
+
+    fields = indicator.CoastlineFields(coastline_mesh(band=7000.0))
+    t0 = time.perf_counter()
+    Xtr, ytr, _ = indicator.build_dataset(fields, range(1000, 1050))
+    Xte, yte, ate = indicator.build_dataset(fields, range(5000, 5005))
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    model = mlp.IndicatorMLP(seed=7)
+    t0 = time.perf_counter()
+    loss = indicator.train(model, Xtr, ytr, epochs=6, batch=2048, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    err = indicator.misclassification(model, Xte, ate)
+    n = Xte.shape[0]
+    ms_inf = time_launches(lambda: mlp.classify(model.forward(Xte, stream), indicator.REFINE_LEVEL,
+                                                indicator.COARSEN_LEVEL, stream), 20, 3, stream)
+    est = adapt.GradientErrorEstimator(3)
+    eta, T = fields.sample(5000)
+    eta5, T5 = eta.repeat(5, 1), T.repeat(5, 1)
+    h5 = fields.h_char.repeat(5)
+    ms_est = time_launches(lambda: (est.estimate(eta5, h5), est.estimate(T5, h5)), 20, 3, stream)
+    counts = torch.bincount(ate.long(), minlength=3).cpu().tolist()
+    log(f"indicator: {Xtr.shape[0]} training rows, misclassification {err:.4f} on {n} "
+        f"unseen elements {counts}; inference {ms_inf * 1e3:.1f} us vs estimator "
+        f"{ms_est * 1e3:.1f} us; train {t_train:.1f} s")
+    return {
+        "indicator_training_rows": int(Xtr.shape[0]),
+        "indicator_train_s": round(t_train, 2),
+        "indicator_data_s": round(t_data, 2),
+        "indicator_final_train_loss": round(loss, 5),
+        "indicator_misclassification_unseen": round(err, 4),
+        "indicator_unseen_decisions_none_refine_coarsen": counts,
+        "indicator_inference_us_196400_elements": round(ms_inf * 1e3, 1),
+        "estimator_two_fields_us_196400_elements": round(ms_est * 1e3, 1),
     }
 
 
@@ -194,6 +285,9 @@ def main():
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
     R = torch.empty_like(U)
+    # launch shape from the library's autotuner (workgroups per CU, XCD order)
+    tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
+    log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
     ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
                        args.steps, args.warmup, stream)
     gdof = dof * 5 / (ms * 1e-3) / 1e9
@@ -204,7 +298,9 @@ def main():
 
     extras = None
     if not args.rhs_only:
-        extras = measure_extras(args, case, s64, U, r64, stream)
+        extras = measure_extras(args, case, s64, U, r64, stream, tune)
+        extras.update(measure_coastline(args, stream))
+        extras.update(measure_indicator(stream))
 
     cpu = None if args.no_cpu else cpu_baseline(case)
     if cpu:

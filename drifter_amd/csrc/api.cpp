@@ -151,7 +151,7 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c2 = c2;
         epi.c3 = c3;
         VolGeom<double> geo{c->E, coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
-                            c->f0};
+                            c->f0, c->wg_per_cu, c->xcd_remap};
         err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
                                  s);
     } else {
@@ -168,7 +168,7 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c2 = c2;
         epi.c3 = c3;
         VolGeom<float> geo{c->E, coef_view<float>(c->d_h32, c), coef_view<float>(c->d_f32, c),
-                           c->f0};
+                           c->f0, c->wg_per_cu, c->xcd_remap};
         err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
                                  s);
     }
@@ -531,6 +531,80 @@ drf_status drf_rhs_dev(drf_ctx *c, double t, const void *U, void *dUdt, int layo
     DRF_REQUIRE(U && dUdt, DRF_ERR_ARG, "null state pointer");
     DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
     return rhs_into(c, U, dUdt, layout, pick_stream(c, stream));
+}
+
+drf_status drf_set_launch(drf_ctx *c, int wg_per_cu, int xcd_remap)
+{
+    DRF_REQUIRE(c, DRF_ERR_ARG, "null context");
+    DRF_REQUIRE(wg_per_cu >= 0 && wg_per_cu <= 64, DRF_ERR_ARG, "wg_per_cu must be 0..64");
+    c->wg_per_cu = wg_per_cu;
+    c->xcd_remap = xcd_remap ? 1 : 0;
+    return DRF_OK;
+}
+
+drf_status drf_get_launch(const drf_ctx *c, int *wg_per_cu, int *xcd_remap)
+{
+    DRF_REQUIRE(c && wg_per_cu && xcd_remap, DRF_ERR_ARG, "null argument");
+    *wg_per_cu = c->wg_per_cu;
+    *xcd_remap = c->xcd_remap;
+    return DRF_OK;
+}
+
+drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int reps,
+                        void *stream, float *best_us)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && dUdt, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
+    DRF_REQUIRE(reps >= 1 && reps <= 1000, DRF_ERR_ARG, "reps must be 1..1000");
+    DRF_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = pick_stream(c, stream);
+    hipEvent_t e0, e1;
+    DRF_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(DRF_ERR_HIP, "hipEventCreate failed");
+    }
+    // resident workgroups per CU (values above the occupancy limit clamp to
+    // it) x XCD-aware batch order
+    static const int kWpc[] = {1, 2, 3, 4, 5, 6, 8};
+    float best = 1e30f;
+    int bw = c->wg_per_cu, br = c->xcd_remap;
+    drf_status res = DRF_OK;
+    for (int remap = 0; remap < 2 && res == DRF_OK; ++remap)
+        for (int wpc : kWpc) {
+            c->wg_per_cu = wpc;
+            c->xcd_remap = remap;
+            res = rhs_into(c, U, dUdt, layout, s); // warm-up
+            for (int r = 0; r < reps && res == DRF_OK; ++r) {
+                if (r == 0 && hipEventRecord(e0, s) != hipSuccess)
+                    res = fail(DRF_ERR_HIP, "hipEventRecord failed");
+                if (res == DRF_OK)
+                    res = rhs_into(c, U, dUdt, layout, s);
+            }
+            float ms = 0.f;
+            if (res == DRF_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                                  hipEventSynchronize(e1) != hipSuccess ||
+                                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                res = fail(DRF_ERR_HIP, "event timing failed");
+            if (res != DRF_OK)
+                break;
+            const float us = ms * 1000.f / reps;
+            if (us < best) {
+                best = us;
+                bw = wpc;
+                br = remap;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->wg_per_cu = bw;
+    c->xcd_remap = br;
+    if (best_us)
+        *best_us = best;
+    return res;
 }
 
 drf_status drf_rhs(drf_ctx *c, double t, const double *U, double *dUdt)

@@ -104,6 +104,8 @@ template <class T> struct VolGeom {
     View<const T> h; // bathymetry at every node (fs unused)
     View<const T> f; // Coriolis parameter at every node; f.p == nullptr -> f0
     double f0;
+    int wg_per_cu;   // launch shape: resident workgroups per CU (0: occupancy limit)
+    int xcd_remap;   // renumber workgroups so neighbouring batches share an XCD
 };
 
 // Launchers (volume.hip).  Return hipError_t of the launch.
@@ -150,4 +152,7 @@ struct drf_ctx {
     size_t work_bytes = 0;
     double *d_red = nullptr; // reduction scratch
     int64_t red_len = 0;
+    // volume-kernel launch shape (drf_set_launch / drf_autotune)
+    int wg_per_cu = 0;
+    int xcd_remap = 1;
 };

@@ -169,8 +169,9 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
     // being line-aligned -- behind the same L2, while the grid as a whole
     // still sweeps memory in one contiguous window per round.
     const int64_t G = gridDim.x;
-    const int64_t lw = (G % kXcds == 0) ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
-                                        : (int64_t)blockIdx.x;
+    const int64_t lw = (geo.xcd_remap && G % kXcds == 0)
+                           ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
+                           : (int64_t)blockIdx.x;
     int64_t b = lw;
     Lines<n, TC> nxt;
     if (PREFETCH) {
@@ -545,7 +546,8 @@ hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, Vie
     }();
     // persistent workgroups: a few rounds per workgroup so the next batch's
     // loads overlap the current batch's arithmetic
-    int64_t grid = (int64_t)num_cus() * per_cu;
+    const int wpc = (geo.wg_per_cu > 0 && geo.wg_per_cu < per_cu) ? geo.wg_per_cu : per_cu;
+    int64_t grid = (int64_t)num_cus() * wpc;
     if (grid > nbatch)
         grid = nbatch;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(C::BLOCK), 0, s, geo,

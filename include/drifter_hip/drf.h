@@ -111,6 +111,16 @@ drf_status drf_rhs(drf_ctx *ctx, double t, const double *U, double *dUdt);
 drf_status drf_rhs_dev(drf_ctx *ctx, double t, const void *U, void *dUdt, int layout,
                        void *stream);
 
+/* Launch shape of the volume kernel (no counterpart in the reference):
+ * resident workgroups per CU (0 = the occupancy limit, the default) and
+ * the XCD-aware batch order (1, the default).  drf_autotune times
+ * drf_rhs_dev on U for each shape (reps launches each, on `stream`), keeps
+ * the fastest in the context and returns its time per launch. */
+drf_status drf_set_launch(drf_ctx *ctx, int wg_per_cu, int xcd_remap);
+drf_status drf_get_launch(const drf_ctx *ctx, int *wg_per_cu, int *xcd_remap);
+drf_status drf_autotune(drf_ctx *ctx, const void *U, void *dUdt, int layout, int reps,
+                        void *stream, float *best_us);
+
 /* ------------------------------------------------------------------ */
 /* Time stepping (TimeStepper::step, include/solver/time_stepper.hpp:18) */
 /* ------------------------------------------------------------------ */
