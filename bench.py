@@ -158,7 +158,8 @@ def measure_extras(args, case, s64, U, r64, stream, tune=None):
             steps_err[k] = [float("%.3e" % x) for x in e]
     log(f"mixed vs FP64 rel L2 after 1 / 100 RK4 steps: {steps_err[1]} / {steps_err[100]}")
     return {
-        "autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1], "us": round(tune[2], 2)}
+        "autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1],
+                          "us": round(tune[2], 2) if tune[2] == tune[2] else None}
         if tune else None,
         "autotune_mixed": {"wg_per_cu": tune32[0], "xcd_remap": tune32[1],
                            "us": round(tune32[2], 2)},
@@ -223,17 +224,25 @@ def measure_indicator(stream):
     t_train = time.perf_counter() - t0
     err = indicator.misclassification(model, Xte, ate)
     n = Xte.shape[0]
-    ms_inf = time_launches(lambda: mlp.classify(model.forward(Xte, stream), indicator.REFINE_LEVEL,
-                                                indicator.COARSEN_LEVEL, stream), 20, 3, stream)
     est = adapt.GradientErrorEstimator(3)
     eta, T = fields.sample(5000)
-    eta5, T5 = eta.repeat(5, 1), T.repeat(5, 1)
-    h5 = fields.h_char.repeat(5)
+    eta5, T5 = eta.repeat(5, 1).contiguous(), T.repeat(5, 1).contiguous()
+    h5 = fields.h_char.repeat(5).contiguous()
+    X5 = torch.zeros((n, 128), dtype=torch.int16, device="cuda")
+
+    def infer():  # features of both fields from the nodal state, forward, decisions
+        adapt.modal_features_bf16(3, eta5, n, X5, 0, elem_scale=h5, stream=stream)
+        adapt.modal_features_bf16(3, T5, n, X5, 64, elem_scale=h5, stream=stream)
+        return mlp.classify(model.forward(X5, stream), indicator.REFINE_LEVEL,
+                            indicator.COARSEN_LEVEL, stream)
+
+    ms_inf = time_launches(infer, 20, 3, stream)
+    ms_fwd = time_launches(lambda: model.forward(Xte, stream), 20, 3, stream)
     ms_est = time_launches(lambda: (est.estimate(eta5, h5), est.estimate(T5, h5)), 20, 3, stream)
     counts = torch.bincount(ate.long(), minlength=3).cpu().tolist()
     log(f"indicator: {Xtr.shape[0]} training rows, misclassification {err:.4f} on {n} "
-        f"unseen elements {counts}; inference {ms_inf * 1e3:.1f} us vs estimator "
-        f"{ms_est * 1e3:.1f} us; train {t_train:.1f} s")
+        f"unseen elements {counts}; features + inference {ms_inf * 1e3:.1f} us (forward "
+        f"{ms_fwd * 1e3:.1f} us) vs estimator {ms_est * 1e3:.1f} us; train {t_train:.1f} s")
     return {
         "indicator_training_rows": int(Xtr.shape[0]),
         "indicator_train_s": round(t_train, 2),
@@ -241,7 +250,8 @@ def measure_indicator(stream):
         "indicator_final_train_loss": round(loss, 5),
         "indicator_misclassification_unseen": round(err, 4),
         "indicator_unseen_decisions_none_refine_coarsen": counts,
-        "indicator_inference_us_196400_elements": round(ms_inf * 1e3, 1),
+        "indicator_features_and_inference_us_196400_elements": round(ms_inf * 1e3, 1),
+        "indicator_forward_us_196400_elements": round(ms_fwd * 1e3, 1),
         "estimator_two_fields_us_196400_elements": round(ms_est * 1e3, 1),
     }
 
@@ -252,6 +262,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="WPC,REMAP",
+                    help="fixed volume-kernel launch shape (workgroups per CU, XCD order 0/1) "
+                         "instead of the autotuner's choice")
     ap.add_argument("--rhs-only", action="store_true",
                     help="time the FP64 RHS only (no mixed-precision or RK extras)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -286,8 +299,14 @@ def main():
     U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
     R = torch.empty_like(U)
     # launch shape from the library's autotuner (workgroups per CU, XCD order)
-    tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
-    log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
+    if args.launch:
+        wpc, remap = (int(v) for v in args.launch.split(","))
+        s64.set_launch(wpc, remap)
+        tune = (wpc, bool(remap), float("nan"))
+        log(f"launch shape fixed: {wpc} workgroups/CU, xcd_remap={bool(remap)}")
+    else:
+        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
+        log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
     ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
                        args.steps, args.warmup, stream)
     gdof = dof * 5 / (ms * 1e-3) / 1e9

@@ -57,6 +57,10 @@ SIGNATURES = {
     "drf_compute_dt": (_D, [_D, _D, _D, _D, _D]),
     "drf_mode_split_step": (_I, [_P, _D, _I, _PD]),
     "drf_mode_split_step_dev": (_I, [_P, _D, _I, _P, _P]),
+    "drf_graph_rk": (_I, [_P, _I, _D, _P, _I, _I, _I, C.POINTER(_P)]),
+    "drf_graph_mode_split": (_I, [_P, _D, _I, _P, _I, C.POINTER(_P)]),
+    "drf_graph_launch": (_I, [_P, _P]),
+    "drf_graph_destroy": (None, [_P]),
     "drf_mesh_create": (_I, [_D, _D, _D, _D, _D, _D, C.POINTER(_P)]),
     "drf_mesh_destroy": (None, [_P]),
     "drf_mesh_build_uniform": (_I, [_P, _I, _I, _I]),

@@ -4,11 +4,13 @@
 // Replaces GradientErrorEstimator::estimate / estimate_element /
 // compute_directional_mask (src/amr/refinement.cpp:44-161) and the marking
 // loop of AdaptiveMeshRefinement::mark_elements (:488-520, can_refine /
-// can_coarsen :598-606).  One wavefront per element, grid-stride over
-// elements; the element's nodes are staged in the wave's slice of LDS so
-// that line variations along xi, eta and zeta read LDS, not HBM.  Every
-// value is a max, min, difference or single product, so results are
-// bit-identical to the sequential reference loop.
+// can_coarsen :598-606).  A wavefront takes 64 / P elements at a time (P =
+// n^2 rounded up to a power of two, at most 64): their nodes are staged in
+// the wave's slice of LDS with coalesced loads, and each lane owns grid lines
+// of one element, so line variations along xi, eta and zeta read LDS and the
+// per-element reductions stay within a group of P lanes.  Every value is a
+// max, min, difference or single product, so results are bit-identical to
+// the sequential reference loop.
 #include "adapt.hpp"
 #include "drf_internal.hpp"
 
@@ -26,11 +28,42 @@ __device__ __forceinline__ double wave_reduce_max(double v)
         v = dmax(v, __shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ double wave_reduce_min(double v)
+
+// Lanes of a wavefront are split into groups of P = min(64, 2^ceil(log2 n^2))
+// lanes, one group per element; lane L of a group owns grid lines L, L + P, ...
+// (< n^2) of its element.  Element values are staged in the wave's LDS slice
+// with coalesced loads.
+struct LineMap {
+    int P, epw, lpl; // group size, elements per wave, lines per lane
+};
+
+__host__ __device__ inline LineMap line_map(int n)
 {
-    for (int o = 32; o > 0; o >>= 1)
+    const int n2 = n * n;
+    int P = 1;
+    while (P < n2 && P < 64)
+        P <<= 1;
+    return LineMap{P, 64 / P, (n2 + P - 1) / P};
+}
+
+__device__ __forceinline__ double group_max(double v, int P)
+{
+    for (int o = P >> 1; o > 0; o >>= 1)
+        v = dmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double group_min(double v, int P)
+{
+    for (int o = P >> 1; o > 0; o >>= 1)
         v = dmin(v, __shfl_xor(v, o));
     return v;
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Pass 1: raw indicator value[e] = (max - min) * h_char[e] and the
@@ -44,69 +77,79 @@ __global__ void __launch_bounds__(64 * kWaves)
     __shared__ double red[kWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n2 = n * n, nd = n2 * n;
-    double *u = sm + (size_t)w * nd;
+    const LineMap lm = line_map(n);
+    const int g = lane / lm.P, gl = lane % lm.P; // element within the wave, lane within it
+    double *u = sm + (size_t)w * lm.epw * nd;
     double bmax = 0.0;
-    for (int64_t e = (int64_t)blockIdx.x * kWaves + w; e < E; e += (int64_t)gridDim.x * kWaves) {
-        const double *src = field + e * es;
-        double hi = -INFINITY, lo = INFINITY;
-        for (int d = lane; d < nd; d += 64) {
-            const double v = src[d];
-            u[d] = v;
-            hi = dmax(hi, v);
-            lo = dmin(lo, v);
+    const int64_t waves = (int64_t)gridDim.x * kWaves;
+    for (int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw; e0 < E; e0 += waves * lm.epw) {
+        const int cnt = (E - e0 < lm.epw) ? (int)(E - e0) : lm.epw;
+        for (int idx = lane; idx < cnt * nd; idx += 64) {
+            const int el = idx / nd, d = idx - el * nd;
+            u[idx] = field[(e0 + el) * es + d];
         }
-        hi = wave_reduce_max(hi);
-        lo = wave_reduce_min(lo);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // lines: 0..n2-1 along xi, n2..2n2-1 along eta, 2n2..3n2-1 along zeta
-        double vx = 0.0, vy = 0.0, vz = 0.0;
-        for (int L = lane; L < 3 * n2; L += 64) {
-            const int dir = L / n2, r = L - dir * n2;
-            const int a = r % n, b = r / n;
-            int base, stride;
-            if (dir == 0) { // (j, k) = (a, b)
-                base = b * n2 + a * n;
-                stride = 1;
-            } else if (dir == 1) { // (i, k) = (a, b)
-                base = b * n2 + a;
-                stride = n;
-            } else { // (i, j) = (a, b)
-                base = b * n + a;
-                stride = n2;
+        wave_sync_lds();
+        double hi = -INFINITY, lo = INFINITY, vx = 0.0, vy = 0.0, vz = 0.0;
+        if (g < cnt) {
+            const double *ue = u + g * nd;
+            for (int t = 0; t < lm.lpl; ++t) {
+                const int L = gl + t * lm.P;
+                if (L >= n2)
+                    break;
+                const int a = L % n, b = L / n;
+                // xi-line (j, k) = (a, b): also the element's max and min
+                double l0 = ue[b * n2 + a * n], h0 = l0;
+                for (int i = 1; i < n; ++i) {
+                    const double v = ue[b * n2 + a * n + i];
+                    l0 = dmin(l0, v);
+                    h0 = dmax(h0, v);
+                }
+                vx = dmax(vx, h0 - l0);
+                hi = dmax(hi, h0);
+                lo = dmin(lo, l0);
+                // eta-line (i, k) = (a, b)
+                l0 = ue[b * n2 + a];
+                h0 = l0;
+                for (int j = 1; j < n; ++j) {
+                    const double v = ue[b * n2 + j * n + a];
+                    l0 = dmin(l0, v);
+                    h0 = dmax(h0, v);
+                }
+                vy = dmax(vy, h0 - l0);
+                // zeta-line (i, j) = (a, b)
+                l0 = ue[b * n + a];
+                h0 = l0;
+                for (int k = 1; k < n; ++k) {
+                    const double v = ue[k * n2 + b * n + a];
+                    l0 = dmin(l0, v);
+                    h0 = dmax(h0, v);
+                }
+                vz = dmax(vz, h0 - l0);
             }
-            double l0 = u[base], h0 = l0;
-            for (int t = 1; t < n; ++t) {
-                const double v = u[base + t * stride];
-                l0 = dmin(l0, v);
-                h0 = dmax(h0, v);
-            }
-            const double var = h0 - l0;
-            if (dir == 0)
-                vx = dmax(vx, var);
-            else if (dir == 1)
-                vy = dmax(vy, var);
-            else
-                vz = dmax(vz, var);
         }
-        vx = wave_reduce_max(vx);
-        vy = wave_reduce_max(vy);
-        vz = wave_reduce_max(vz);
-        const double val = (hi - lo) * hchar[e];
-        bmax = dmax(bmax, val);
-        if (lane == 0) {
-            const double mv = dmax(dmax(vx, vy), vz);
-            uint8_t m = 0;
-            if (!(mv < 1e-14)) {
-                const double thr = smooth * mv;
-                m = (vx > thr ? 1 : 0) | (vy > thr ? 2 : 0) | (vz > thr ? 4 : 0);
+        hi = group_max(hi, lm.P);
+        lo = group_min(lo, lm.P);
+        vx = group_max(vx, lm.P);
+        vy = group_max(vy, lm.P);
+        vz = group_max(vz, lm.P);
+        if (g < cnt) {
+            const int64_t e = e0 + g;
+            const double val = (hi - lo) * hchar[e];
+            bmax = dmax(bmax, val);
+            if (gl == 0) {
+                const double mv = dmax(dmax(vx, vy), vz);
+                uint8_t m = 0;
+                if (!(mv < 1e-14)) {
+                    const double thr = smooth * mv;
+                    m = (vx > thr ? 1 : 0) | (vy > thr ? 2 : 0) | (vz > thr ? 4 : 0);
+                }
+                value[e] = val;
+                mask[e] = m;
             }
-            value[e] = val;
-            mask[e] = m;
         }
-        __builtin_amdgcn_wave_barrier(); // LDS slice reused by the next element
+        wave_sync_lds(); // the slice is reloaded for the next elements
     }
+    bmax = group_max(bmax, 64);
     if (lane == 0)
         red[w] = bmax;
     __syncthreads();
@@ -117,6 +160,7 @@ __global__ void __launch_bounds__(64 * kWaves)
         partial[blockIdx.x] = m;
     }
 }
+
 
 // Pass 2: global max from the partials (every block), normalisation,
 // thresholds, then mark_elements when levels are given.
@@ -178,9 +222,10 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
     return (uint16_t)(x >> 16);
 }
 
-// Nodal -> modal by sum factorisation: three passes of n x n contractions
-// (xi, then eta, then zeta) through the wave's LDS slice; one wavefront per
-// element.  OUT = double (coefficients) or uint16_t (BF16 features, scaled).
+// Nodal -> modal by sum factorisation, line-parallel: three passes of n x n
+// contractions (xi, then eta, then zeta) through the wave's LDS slice, lanes
+// owning grid lines.  OUT = double (coefficients) or uint16_t (BF16
+// features, scaled, mean mode zeroed).
 template <class OUT>
 __global__ void __launch_bounds__(64 * kWaves)
     modal_kernel(int n, int64_t E, const double *__restrict__ u, int64_t es, ModalOps ops,
@@ -190,55 +235,63 @@ __global__ void __launch_bounds__(64 * kWaves)
     extern __shared__ double sm[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n2 = n * n, nd = n2 * n;
-    double *a = sm + (size_t)w * 2 * nd, *b = a + nd;
-    for (int64_t e = (int64_t)blockIdx.x * kWaves + w; e < E; e += (int64_t)gridDim.x * kWaves) {
-        const double *src = u + e * es;
-        for (int d = lane; d < nd; d += 64)
-            a[d] = src[d];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int d = lane; d < nd; d += 64) { // b[p + n(j + n k)] = sum_i Vi[p][i] a[i + ...]
-            const int p = d % n, jk = d / n;
-            double s = 0.0;
-            for (int i = 0; i < n; ++i)
-                s += ops.Vinv[p * kMaxN + i] * a[jk * n + i];
-            b[d] = s;
+    const LineMap lm = line_map(n);
+    const int g = lane / lm.P, gl = lane % lm.P;
+    double *a = sm + (size_t)w * 2 * lm.epw * nd, *bb = a + lm.epw * nd;
+    const int64_t waves = (int64_t)gridDim.x * kWaves;
+    for (int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw; e0 < E; e0 += waves * lm.epw) {
+        const int cnt = (E - e0 < lm.epw) ? (int)(E - e0) : lm.epw;
+        for (int idx = lane; idx < cnt * nd; idx += 64) {
+            const int el = idx / nd, d = idx - el * nd;
+            a[idx] = u[(e0 + el) * es + d];
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int d = lane; d < nd; d += 64) { // a[p + n(q + n k)] = sum_j Vi[q][j] b[p + n(j + n k)]
-            const int p = d % n, q = (d / n) % n, k = d / n2;
-            double s = 0.0;
-            for (int j = 0; j < n; ++j)
-                s += ops.Vinv[q * kMaxN + j] * b[k * n2 + j * n + p];
-            a[d] = s;
+        wave_sync_lds();
+        const bool act = g < cnt;
+        double *ae = a + g * nd, *be = bb + g * nd;
+        if (act) // xi: b[p + n L] = sum_i Vi[p][i] a[i + n L], L = (j, k)
+            for (int L = gl; L < n2; L += lm.P)
+                for (int p = 0; p < n; ++p) {
+                    double s = 0.0;
+                    for (int i = 0; i < n; ++i)
+                        s += ops.Vinv[p * kMaxN + i] * ae[L * n + i];
+                    be[L * n + p] = s;
+                }
+        wave_sync_lds();
+        if (act) // eta: a[p + n(q + n k)] = sum_j Vi[q][j] b[p + n(j + n k)], L = (p, k)
+            for (int L = gl; L < n2; L += lm.P) {
+                const int p = L % n, k = L / n;
+                for (int q = 0; q < n; ++q) {
+                    double s = 0.0;
+                    for (int j = 0; j < n; ++j)
+                        s += ops.Vinv[q * kMaxN + j] * be[k * n2 + j * n + p];
+                    ae[k * n2 + q * n + p] = s;
+                }
+            }
+        wave_sync_lds();
+        if (act) { // zeta: coef[L + n2 r] = sum_k Vi[r][k] a[L + n2 k], L = (p, q)
+            const int64_t e = e0 + g;
+            OUT *dst = out + e * out_stride + out_offset;
+            const double sc = elem_scale ? scale * elem_scale[e] : scale;
+            for (int L = gl; L < n2; L += lm.P)
+                for (int r = 0; r < n; ++r) {
+                    double s = 0.0;
+                    for (int k = 0; k < n; ++k)
+                        s += ops.Vinv[r * kMaxN + k] * ae[k * n2 + L];
+                    const int d = L + n2 * r;
+                    if constexpr (sizeof(OUT) == 8)
+                        dst[d] = s;
+                    else // features: the element mean (mode 0) carries no variation
+                        dst[d] = (d == 0) ? (uint16_t)0 : to_bf16((float)(s * sc));
+                }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        OUT *dst = out + e * out_stride + out_offset;
-        const double sc = elem_scale ? scale * elem_scale[e] : scale;
-        for (int d = lane; d < nd; d += 64) { // coef[p + n(q + n r)] = sum_k Vi[r][k] a[p + n(q + n k)]
-            const int pq = d % n2, r = d / n2;
-            double s = 0.0;
-            for (int k = 0; k < n; ++k)
-                s += ops.Vinv[r * kMaxN + k] * a[k * n2 + pq];
-            if constexpr (sizeof(OUT) == 8)
-                dst[d] = s;
-            else // features: the element mean (mode 0) carries no variation
-                dst[d] = (d == 0) ? (uint16_t)0 : to_bf16((float)(s * sc));
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
     }
 }
 
-unsigned grid_for(int64_t E, int64_t cap)
+unsigned grid_for(int n, int64_t E, int64_t cap)
 {
-    const int64_t need = (E + kWaves - 1) / kWaves;
+    const int64_t per_block = (int64_t)kWaves * line_map(n).epw;
+    const int64_t need = (E + per_block - 1) / per_block;
     return (unsigned)(need < cap ? need : cap);
 }
 
@@ -251,8 +304,8 @@ hipError_t launch_estimate(int n, int64_t E, const double *field, int64_t es, co
 {
     if (E <= 0)
         return hipSuccess;
-    const unsigned g1 = grid_for(E, kEstimatorPartials);
-    const size_t lds = sizeof(double) * (size_t)kWaves * n * n * n;
+    const unsigned g1 = grid_for(n, E, kEstimatorPartials);
+    const size_t lds = sizeof(double) * (size_t)kWaves * line_map(n).epw * n * n * n;
     hipLaunchKernelGGL(estimate_kernel, dim3(g1), dim3(64 * kWaves), lds, s, n, E, field, es, hchar,
                        p.smoothness_factor, value, mask, partial);
     hipError_t err = hipGetLastError();
@@ -271,8 +324,8 @@ hipError_t launch_modal(int n, int64_t E, const double *u, int64_t es, const Mod
 {
     if (E <= 0)
         return hipSuccess;
-    const unsigned g = grid_for(E, 8192);
-    const size_t lds = sizeof(double) * (size_t)kWaves * 2 * n * n * n;
+    const unsigned g = grid_for(n, E, 8192);
+    const size_t lds = sizeof(double) * (size_t)kWaves * 2 * line_map(n).epw * n * n * n;
     if (coef)
         hipLaunchKernelGGL(modal_kernel<double>, dim3(g), dim3(64 * kWaves), lds, s, n, E, u, es,
                            ops, coef, out_stride, out_offset, scale, elem_scale);

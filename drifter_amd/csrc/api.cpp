@@ -375,6 +375,89 @@ drf_status wave_speed(drf_ctx *c, const void *U, int layout, double *out, hipStr
 
 } // namespace
 
+
+// A captured sequence of steps.  The graph owns the workspace its kernels
+// use, so later growth of the context's own workspace cannot leave it
+// pointing at freed memory.
+struct drf_graph {
+    int device;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    void *work;
+};
+
+drf_status drf_graph_destroy_impl(drf_graph *g);
+
+namespace {
+
+template <class Body>
+drf_status capture_graph(drf_ctx *c, size_t work_bytes, int n_steps, drf_graph **out, Body body)
+{
+    DRF_HIP_CHECK(hipSetDevice(c->device));
+    drf_graph *g = new drf_graph{c->device, nullptr, nullptr, nullptr};
+    if (work_bytes && hipMalloc(&g->work, work_bytes) != hipSuccess) {
+        delete g;
+        return fail(DRF_ERR_HIP, "graph workspace allocation failed");
+    }
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(g->work);
+        delete g;
+        return fail(DRF_ERR_HIP, "capture stream creation failed");
+    }
+    // route the context's workspace to the graph's own buffer while capturing
+    void *saved = c->d_work;
+    const size_t saved_bytes = c->work_bytes;
+    if (work_bytes) {
+        c->d_work = g->work;
+        c->work_bytes = work_bytes;
+    }
+    drf_status st = DRF_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess)
+        st = fail(DRF_ERR_HIP, std::string("begin capture: ") + hipGetErrorString(e));
+    for (int i = 0; i < n_steps && st == DRF_OK && e == hipSuccess; ++i)
+        st = body(cs);
+    if (e == hipSuccess) {
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+        if (st == DRF_OK && e2 != hipSuccess)
+            st = fail(DRF_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e2));
+        g->graph = graph;
+    }
+    c->d_work = saved;
+    c->work_bytes = saved_bytes;
+    (void)hipStreamDestroy(cs);
+    if (st == DRF_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess)
+            st = fail(DRF_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    if (st != DRF_OK) {
+        drf_graph_destroy_impl(g);
+        return st;
+    }
+    *out = g;
+    return DRF_OK;
+}
+
+} // namespace
+
+drf_status drf_graph_destroy_impl(drf_graph *g)
+{
+    if (!g)
+        return DRF_OK;
+    (void)hipSetDevice(g->device);
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    if (g->work)
+        (void)hipFree(g->work);
+    delete g;
+    return DRF_OK;
+}
+
 extern "C" {
 
 void drf_default_physics(drf_physics *p)
@@ -784,5 +867,62 @@ double drf_rhs_bytes(const drf_ctx *c)
 
 int64_t drf_num_elements(const drf_ctx *c) { return c ? c->E : 0; }
 int drf_order(const drf_ctx *c) { return c ? c->order : 0; }
+
+// ---------------- HIP graphs of repeated steps ----------------
+
+drf_status drf_graph_rk(drf_ctx *c, int scheme, double dt, void *U, int layout, int fused,
+                        int n_steps, drf_graph **out)
+{
+    DRF_REQUIRE(out, DRF_ERR_ARG, "null output");
+    *out = nullptr;
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
+    DRF_REQUIRE(scheme >= DRF_RK_EULER && scheme <= DRF_RK_DP45, DRF_ERR_ARG,
+                "unknown Runge-Kutta scheme " + std::to_string(scheme));
+    DRF_REQUIRE(n_steps >= 1 && n_steps <= 100000, DRF_ERR_ARG, "n_steps must be 1..100000");
+    DRF_REQUIRE(fused || c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "un-fused stepping is the FP64 baseline only");
+    const bool uses_lincomb = !fused || scheme == DRF_RK_DP45;
+    const size_t bytes = uses_lincomb ? sizeof(double) * state_len(c) * 7
+                                      : elem_size(c) * state_len(c) * 2;
+    return capture_graph(c, bytes, n_steps, out, [&](hipStream_t cs) {
+        return fused ? rk_fused(c, scheme, dt, U, layout, cs)
+                     : rk_unfused(c, scheme, dt, (double *)U, layout, cs);
+    });
+}
+
+drf_status drf_graph_mode_split(drf_ctx *c, double dt_3d, int n_sub, void *full, int n_steps,
+                                drf_graph **out)
+{
+    DRF_REQUIRE(out, DRF_ERR_ARG, "null output");
+    *out = nullptr;
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(full, DRF_ERR_ARG, "null state pointer");
+    DRF_REQUIRE(n_sub >= 1, DRF_ERR_ARG, "subcycles must be >= 1");
+    DRF_REQUIRE(n_steps >= 1 && n_steps <= 100000, DRF_ERR_ARG, "n_steps must be 1..100000");
+    return capture_graph(c, 0, n_steps, out, [&](hipStream_t cs) -> drf_status {
+        hipError_t e = launch_mode_split(c->order, c->E, dt_3d, n_sub, (double *)full,
+                                         coef_view<double>(c->d_h, c),
+                                         coef_view<double>(c->d_f, c), c->f0, c->ops, c->phys, cs);
+        if (e != hipSuccess)
+            return fail(DRF_ERR_HIP, std::string("mode split launch: ") + hipGetErrorString(e));
+        return DRF_OK;
+    });
+}
+
+drf_status drf_graph_launch(drf_graph *g, void *stream)
+{
+    DRF_REQUIRE(g, DRF_ERR_ARG, "null graph");
+    DRF_HIP_CHECK(hipSetDevice(g->device));
+    DRF_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return DRF_OK;
+}
+
+void drf_graph_destroy(drf_graph *g) { (void)drf_graph_destroy_impl(g); }
 
 } // extern "C"

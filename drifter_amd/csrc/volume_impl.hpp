@@ -12,21 +12,15 @@
 // host: 18 one-dimensional contractions per node instead of 26.
 //
 // Work decomposition.  A workgroup owns EPB elements per batch; every element
-// is handled by n^2 threads, each owning one grid line of n nodes:
-//   phase 1  thread (j,k) owns the xi-line   -> derived state, all
-//            xi-derivatives in registers, the xi-column integrals of the eta
-//            equation, and stages what the other directions need in LDS;
-//   phase 2  thread (i,k) owns the eta-line  -> eta-derivatives from LDS,
-//            written in place over the consumed fluxes;
-//   phase 3  thread (i,j) owns the zeta-line -> zeta second derivatives,
-//            added in place;
-//   phase 4  thread (j,k) again              -> adds the staged contributions
-//            to its register partials and writes the tendency (or a fused
-//            Runge-Kutta stage update) to HBM.
-// Each value crosses LDS once per direction instead of once per derivative
-// tap.  Workgroups are persistent: while a batch is computed, the state lines
-// of the workgroup's next batch are already being loaded into registers, so
-// HBM traffic overlaps the FP64 arithmetic (occupancy is bounded by LDS).
+// is handled by n^2 threads, each owning one grid line of n nodes.  Line
+// ownership rotates xi -> eta -> zeta -> xi through LDS twice per batch, once
+// for the momentum terms and once for the tracers, so that six staged values
+// per node suffice (phases P1 .. P4b at the kernel).  Each value crosses LDS
+// once per direction instead of once per derivative tap.  Workgroups are
+// persistent: while a batch is computed, the state lines of the workgroup's
+// next batch are already being loaded into registers, so HBM traffic
+// overlaps the FP64 arithmetic; the small LDS footprint lets five N=4
+// workgroups share a CU, which hides the rest of the memory latency.
 //
 // Index quirk reproduced on purpose: the free-surface tendency integrates
 // Hu, Hv over the xi index (d % n) and differentiates the column integrals
@@ -42,20 +36,8 @@
 namespace drf {
 namespace vol {
 
-// Staged LDS fields per element node.
-enum : int {
-    S_FUV = 0, // Hu*v  -> eta/zeta contribution to dHu
-    S_FVV,     // Hv*v  -> ... dHv
-    S_FTV,     // v*HT  -> ... dHT
-    S_FSV,     // v*HS  -> ... dHS
-    S_ETA,
-    S_U,
-    S_V,
-    S_H,
-    S_T,
-    S_S,
-    S_NF
-};
+// LDS slots per element node, used in two rounds (see the kernel).
+enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_NF };
 
 template <int N> struct Cfg {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -65,6 +47,14 @@ template <int N> struct Cfg {
     static constexpr int EPB = epb_thr < epb_lds ? epb_thr : epb_lds;
     static constexpr int NT = EPB * n2;
     static constexpr int BLOCK = ((NT + 63) / 64) * 64;
+    // minimum waves per SIMD the register allocation must allow
+    // (the low-storage stage carries two operand arrays through the
+    // epilogue and gets one wave less)
+    template <class TS, int EPI> static constexpr int wpe()
+    {
+        const int w = sizeof(TS) == 8 ? (N <= 4 ? 2 : 1) : (N <= 4 ? 3 : 2);
+        return (EPI == EPI_LSRK && w > 1) ? w - 1 : w;
+    }
 };
 
 // Operators sized for one order, passed by value (kernel-argument SGPRs).
@@ -138,20 +128,94 @@ __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS>
 
 constexpr int kXcds = 8; // MI355X: 8 XCDs of 32 CUs
 
+// Writes fields [F0, F0 + NFD) of the thread's xi-line: the tendency k (RHS)
+// or a fused Runge-Kutta stage update, whose operand lines are loaded here,
+// field by field, right before they are combined.
+template <int n, int F0, int NFD, class TS, class TC, int EPI>
+__device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int L,
+                                             const TC (&kv)[NFD][n])
+{
+    const TC c0 = (TC)epi.c0, c1 = (TC)epi.c1, c2 = (TC)epi.c2, c3 = (TC)epi.c3;
+    if (EPI == EPI_RHS) {
+        const uint32_t ro = lane_off(epi.R, e, L);
+#pragma unroll
+        for (int f = 0; f < NFD; ++f)
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                sto<TS>(epi.R.p + (F0 + f) * epi.R.fs + i * epi.R.is, ro, (TS)kv[f][i]);
+        return;
+    }
+    const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
+    const View<const TS> V2 = (EPI == EPI_COMB)
+                                  ? epi.X
+                                  : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
+    const bool two = (EPI == EPI_STAGE_ACC || EPI == EPI_RK4FIN || EPI == EPI_COMB ||
+                      (EPI == EPI_LSRK && epi.acc_mode != 1));
+    const uint32_t q1 = lane_off(V1, e, L);
+    const uint32_t q2 = two ? lane_off(V2, e, L) : 0u;
+    const uint32_t yo = lane_off(epi.Y, e, L);
+    const uint32_t ao = epi.A.p ? lane_off(epi.A, e, L) : 0u;
+#pragma unroll
+    for (int f = 0; f < NFD; ++f) {
+        TC o1[n], o2[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            o1[i] = ldo<TS, TC>(V1.p + (F0 + f) * V1.fs + i * V1.is, q1);
+            o2[i] = two ? ldo<TS, TC>(V2.p + (F0 + f) * V2.fs + i * V2.is, q2) : (TC)0;
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            const TC k = kv[f][i];
+            TC y;
+            if (EPI == EPI_STAGE) { // Y = U0 + c1 k ; A = k (acc_mode 1)
+                if (epi.acc_mode == 1)
+                    sto<TS>(epi.A.p + (F0 + f) * epi.A.fs + i * epi.A.is, ao, (TS)k);
+                y = o1[i] + c1 * k;
+            } else if (EPI == EPI_STAGE_ACC) { // A = A + c2 k ; Y = U0 + c1 k
+                sto<TS>(epi.A.p + (F0 + f) * epi.A.fs + i * epi.A.is, ao, (TS)(o2[i] + c2 * k));
+                y = o1[i] + c1 * k;
+            } else if (EPI == EPI_RK4FIN) { // Y = U0 + c1 (A + k)
+                y = o1[i] + c1 * (o2[i] + k);
+            } else if (EPI == EPI_COMB) { // Y = c0 U0 + c1 (X + c2 k)
+                y = c0 * o1[i] + c1 * (o2[i] + c2 * k);
+            } else { // EPI_LSRK: A = c2 A + c1 k ; Y = X + c3 A
+                const TC a = (epi.acc_mode == 1) ? c1 * k : c2 * o2[i] + c1 * k;
+                sto<TS>(epi.A.p + (F0 + f) * epi.A.fs + i * epi.A.is, ao, (TS)a);
+                y = o1[i] + c3 * a;
+            }
+            sto<TS>(epi.Y.p + (F0 + f) * epi.Y.fs + i * epi.Y.is, yo, (TS)y);
+        }
+    }
+}
+
+// The kernel.  Six LDS slots per element node, reused in two rounds
+// (momentum, then tracers), so a workgroup of 5 N=4 elements needs 30 KB of
+// staging plus 4 KB of column integrals, and the CU's occupancy is set by
+// registers (4 workgroups at N=4 FP64) rather than by LDS:
+//
+//   P1  xi-owner   xi-derivatives of the momentum terms (registers), column
+//                  integrals (FP64), stages A=Hu v, B=Hv v, C=eta, D=u, E=v, F=H
+//   P2a eta-owner  eta-derivatives of the momentum terms -> A, B
+//   P3a zeta-owner vertical diffusion of u, v             -> A, B +=
+//   P4a xi-owner   dHu, dHv complete (written out); xi-derivatives of the
+//                  tracer terms; restages A=v HT, B=v HS, C=T, D=S (F=H kept)
+//   P23b eta-owner eta-derivatives of the tracer fluxes    -> A, B
+//        zeta-owner vertical diffusion of T, S (same phase) -> C, D
+//   P4b xi-owner   dHT, dHS and d eta complete (written out)
 template <int N, class TS, class TC, int EPI>
-__global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(Cfg<N>::template wpe<TS, EPI>())))
     volume_kernel(const VolGeom<TS> geo, const KOps<N + 1, TC> ops, const Phys ph,
                   const View<const TS> U, const Epi<TS> epi, const int64_t nbatch)
 {
     using C = Cfg<N>;
-    // Operand lines the epilogue reads (U0, A or X of a fused stage update).
-    constexpr int NOPS = (EPI == EPI_RHS) ? 0 : (EPI == EPI_STAGE ? 1 : 2);
-    // The next-batch prefetch costs 6n registers; with operand lines in flight
-    // as well the kernel would spill, so fused stages do not prefetch.
-    constexpr bool PREFETCH = NOPS == 0;
+    // Fused stages load their operand lines in the epilogue; the next-batch
+    // prefetch is kept for the plain RHS, where registers allow it.
+    constexpr bool PREFETCH = EPI == EPI_RHS;
     constexpr int n = C::n, n2 = C::n2, n3 = C::n3, EPB = C::EPB;
     __shared__ TC sm[EPB][S_NF][n3];
-    __shared__ double bar[EPB][2][n2];
+    // column integrals, double-buffered by batch parity: the last phase of
+    // one batch reads them while the first phase of the next writes
+    __shared__ double bar[2][EPB][2][n2];
 
     const int tid = threadIdx.x;
     const bool in_tile = tid < C::NT;
@@ -180,7 +244,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             load_lines<n, TS, TC>(nxt, U, geo.h, e0, L);
     }
 
-    for (; b < nbatch; b += G) {
+    for (int par = 0; b < nbatch; b += G, par ^= 1) {
         const int64_t e = b * EPB + le;
         const bool act = in_tile && e < geo.E;
         Lines<n, TC> cur;
@@ -188,21 +252,23 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             cur = nxt;
         else if (act)
             load_lines<n, TS, TC>(cur, U, geo.h, e, L);
+        TC *s = &sm[le][0][0];
 
-        // Register partial tendencies of the xi-line owned in phases 1 and 4.
-        TC pHu[n], pHv[n], pHT[n], pHS[n];
+        // Register state carried by the xi-owner: the momentum partials
+        // (P1 -> P4a), HT and HS (P1 -> P4a), the tracer partials (P4a -> P4b).
+        TC pHu[n], pHv[n], ht[n], hs[n];
 
-        // ---------------- phase 1: xi-line (j, k) = (L % n, L / n) ----------------
+        // ---------------- P1: xi-line (j, k) = (L % n, L / n) ----------------
         if (act) {
-            TC H[n], u[n], v[n], T[n], S[n];
+            TC H[n], u[n], v[n];
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 H[i] = cur.et[i] + cur.hb[i];
                 const TC hi = hinv_of(H[i]);
                 u[i] = cur.hu[i] * hi;
                 v[i] = cur.hv[i] * hi;
-                T[i] = cur.ht[i] * hi;
-                S[i] = cur.hs[i] * hi;
+                ht[i] = cur.ht[i];
+                hs[i] = cur.hs[i];
             }
             TC fc[n];
             if (geo.f.p) {
@@ -215,10 +281,9 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 for (int i = 0; i < n; ++i)
                     fc[i] = (TC)geo.f0;
             }
-
-            // xi-derivatives (momentum_advection :275-305, tracer_advection
-            // :377-393, barotropic_pressure_gradient :322-337,
-            // horizontal_diffusion :339-355)
+            // xi-derivatives (momentum_advection :275-305,
+            // barotropic_pressure_gradient :322-337, horizontal_diffusion
+            // :339-355) and Coriolis (:307-320)
             TC x[n], d[n];
 #pragma unroll
             for (int i = 0; i < n; ++i)
@@ -247,35 +312,14 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             for (int i = 0; i < n; ++i)
                 pHv[i] += H[i] * (nu_h * d[i]);
 #pragma unroll
-            for (int i = 0; i < n; ++i)
-                x[i] = u[i] * cur.ht[i];
-            dline<n>(D, x, d);
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-                pHT[i] = -d[i];
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-                x[i] = u[i] * cur.hs[i];
-            dline<n>(D, x, d);
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-                pHS[i] = -d[i];
-
-            // Stage the fields the eta- and zeta-line owners need.
-            TC *s = &sm[le][0][0];
-#pragma unroll
             for (int i = 0; i < n; ++i) {
-                const int nd_ = L * n + i;
-                s[S_FUV * n3 + nd_] = cur.hu[i] * v[i];
-                s[S_FVV * n3 + nd_] = cur.hv[i] * v[i];
-                s[S_FTV * n3 + nd_] = v[i] * cur.ht[i];
-                s[S_FSV * n3 + nd_] = v[i] * cur.hs[i];
-                s[S_ETA * n3 + nd_] = cur.et[i];
-                s[S_U * n3 + nd_] = u[i];
-                s[S_V * n3 + nd_] = v[i];
-                s[S_H * n3 + nd_] = H[i];
-                s[S_T * n3 + nd_] = T[i];
-                s[S_S * n3 + nd_] = S[i];
+                const int m = L * n + i;
+                s[S_A * n3 + m] = cur.hu[i] * v[i];
+                s[S_B * n3 + m] = cur.hv[i] * v[i];
+                s[S_C * n3 + m] = cur.et[i];
+                s[S_D * n3 + m] = u[i];
+                s[S_E * n3 + m] = v[i];
+                s[S_F * n3 + m] = H[i];
             }
             // Column integrals of compute_eta_rhs (:244-254): over the xi index,
             // summed in node order from 0, in FP64 at every precision.
@@ -285,36 +329,32 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 su += ops.sw[i] * (double)cur.hu[i];
                 sv += ops.sw[i] * (double)cur.hv[i];
             }
-            bar[le][0][L] = su;
-            bar[le][1][L] = sv;
+            bar[par][le][0][L] = su;
+            bar[par][le][1][L] = sv;
         }
         __syncthreads();
 
-        TC o1[5][n], o2[5][n];
         if (PREFETCH) {
-            // Next batch of this workgroup, in flight during phases 2-4.
+            // Next batch of this workgroup, in flight during the other phases.
             const int64_t bn = b + G;
             const int64_t en = bn * EPB + le;
             if (in_tile && bn < nbatch && en < geo.E)
                 load_lines<n, TS, TC>(nxt, U, geo.h, en, L);
         }
 
-        // ---------------- phase 2: eta-line (i, k) = (L % n, L / n) ----------------
+        // ---------------- P2a: eta-line (i, k) = (L % n, L / n) ----------------
         if (act) {
             const int i0 = L % n, k0 = L / n;
-            TC *s = &sm[le][0][0];
-            TC fuv[n], fvv[n], ftv[n], fsv[n], et[n], u[n], v[n], H[n];
+            TC fuv[n], fvv[n], et[n], u[n], v[n], H[n];
 #pragma unroll
             for (int j = 0; j < n; ++j) {
                 const int m = i0 + n * j + n2 * k0;
-                fuv[j] = s[S_FUV * n3 + m];
-                fvv[j] = s[S_FVV * n3 + m];
-                ftv[j] = s[S_FTV * n3 + m];
-                fsv[j] = s[S_FSV * n3 + m];
-                et[j] = s[S_ETA * n3 + m];
-                u[j] = s[S_U * n3 + m];
-                v[j] = s[S_V * n3 + m];
-                H[j] = s[S_H * n3 + m];
+                fuv[j] = s[S_A * n3 + m];
+                fvv[j] = s[S_B * n3 + m];
+                et[j] = s[S_C * n3 + m];
+                u[j] = s[S_D * n3 + m];
+                v[j] = s[S_E * n3 + m];
+                H[j] = s[S_F * n3 + m];
             }
             TC d[n], aHu[n], aHv[n];
             dline<n>(D, fuv, d);
@@ -337,161 +377,151 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
 #pragma unroll
             for (int j = 0; j < n; ++j)
                 aHv[j] += H[j] * (nu_h * d[j]);
+#pragma unroll
+            for (int j = 0; j < n; ++j) {
+                const int m = i0 + n * j + n2 * k0;
+                s[S_A * n3 + m] = aHu[j];
+                s[S_B * n3 + m] = aHv[j];
+            }
+        }
+        __syncthreads();
+
+        // ---------------- P3a: zeta-line (i, j) = L, momentum ----------------
+        if (act) {
+            TC u[n], v[n], hi[n];
+#pragma unroll
+            for (int k = 0; k < n; ++k) {
+                const int m = L + n2 * k;
+                u[k] = s[S_D * n3 + m];
+                v[k] = s[S_E * n3 + m];
+                hi[k] = hinv_of(s[S_F * n3 + m]);
+            }
+            // vertical_diffusion :357-375: nu_v * d2/dzeta2 * Hinv
+            TC d[n];
+            dline<n>(D2, u, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_A * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
+            dline<n>(D2, v, d);
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+                s[S_B * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
+        }
+        __syncthreads();
+
+        // ---------------- P4a: xi-line again: momentum done, tracers staged ----------------
+        TC pHT[n], pHS[n];
+        if (act) {
+            TC kv[2][n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int m = L * n + i;
+                kv[0][i] = pHu[i] + s[S_A * n3 + m];
+                kv[1][i] = pHv[i] + s[S_B * n3 + m];
+            }
+            write_fields<n, 0, 2, TS, TC, EPI>(epi, e, L, kv);
+            TC u[n], v[n], hi[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int m = L * n + i;
+                u[i] = s[S_D * n3 + m];
+                v[i] = s[S_E * n3 + m];
+                hi[i] = hinv_of(s[S_F * n3 + m]);
+            }
+            // tracer_advection :377-393 along xi
+            TC x[n], d[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = u[i] * ht[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHT[i] = -d[i];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                x[i] = u[i] * hs[i];
+            dline<n>(D, x, d);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                pHS[i] = -d[i];
+#pragma unroll
+            for (int i = 0; i < n; ++i) {
+                const int m = L * n + i;
+                s[S_A * n3 + m] = v[i] * ht[i];
+                s[S_B * n3 + m] = v[i] * hs[i];
+                s[S_C * n3 + m] = ht[i] * hi[i];
+                s[S_D * n3 + m] = hs[i] * hi[i];
+            }
+        }
+        __syncthreads();
+
+        // ---------------- P23b: eta-line and zeta-line, tracers ----------------
+        // Both roles in one phase: the eta-owner reads and rewrites A, B along
+        // its eta-line, the zeta-owner C, D along its zeta-line, so no two
+        // threads touch the same node of the same slot.
+        if (act) {
+            const int i0 = L % n, k0 = L / n;
+            TC ftv[n], fsv[n];
+#pragma unroll
+            for (int j = 0; j < n; ++j) {
+                const int m = i0 + n * j + n2 * k0;
+                ftv[j] = s[S_A * n3 + m];
+                fsv[j] = s[S_B * n3 + m];
+            }
             TC dT[n], dS[n];
             dline<n>(D, ftv, dT);
             dline<n>(D, fsv, dS);
 #pragma unroll
             for (int j = 0; j < n; ++j) {
                 const int m = i0 + n * j + n2 * k0;
-                s[S_FUV * n3 + m] = aHu[j];
-                s[S_FVV * n3 + m] = aHv[j];
-                s[S_FTV * n3 + m] = -dT[j];
-                s[S_FSV * n3 + m] = -dS[j];
+                s[S_A * n3 + m] = -dT[j];
+                s[S_B * n3 + m] = -dS[j];
             }
-        }
-        __syncthreads();
-
-        // Operand lines of the fused stage update: a single array is in flight
-        // during phase 3; two arrays are loaded after phase 3, when the
-        // zeta-line temporaries are dead (loading them earlier spills).
-        if (NOPS == 1 && act) {
-            const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
-            const bool two = NOPS == 2 && !(EPI == EPI_LSRK && epi.acc_mode == 1);
-            const View<const TS> V2 =
-                (EPI == EPI_COMB) ? epi.X : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
-            const uint32_t q1 = lane_off(V1, e, L);
-#pragma unroll
-            for (int f = 0; f < 5; ++f)
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                    o1[f][i] = ldo<TS, TC>(V1.p + f * V1.fs + i * V1.is, q1);
-            if (two) {
-                const uint32_t q2 = lane_off(V2, e, L);
-#pragma unroll
-                for (int f = 0; f < 5; ++f)
-#pragma unroll
-                    for (int i = 0; i < n; ++i)
-                        o2[f][i] = ldo<TS, TC>(V2.p + f * V2.fs + i * V2.is, q2);
-            }
-        }
-
-        // ---------------- phase 3: zeta-line (i, j) = L ----------------
-        if (act) {
-            TC *s = &sm[le][0][0];
-            TC u[n], v[n], T[n], S[n], hi[n];
+            TC T[n], S[n], hi[n];
 #pragma unroll
             for (int k = 0; k < n; ++k) {
                 const int m = L + n2 * k;
-                u[k] = s[S_U * n3 + m];
-                v[k] = s[S_V * n3 + m];
-                T[k] = s[S_T * n3 + m];
-                S[k] = s[S_S * n3 + m];
-                hi[k] = hinv_of(s[S_H * n3 + m]);
+                T[k] = s[S_C * n3 + m];
+                S[k] = s[S_D * n3 + m];
+                hi[k] = hinv_of(s[S_F * n3 + m]);
             }
-            // vertical_diffusion :357-375: kappa * d2/dzeta2 * Hinv
             TC d[n];
-            dline<n>(D2, u, d);
-#pragma unroll
-            for (int k = 0; k < n; ++k)
-                s[S_FUV * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
-            dline<n>(D2, v, d);
-#pragma unroll
-            for (int k = 0; k < n; ++k)
-                s[S_FVV * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
             dline<n>(D2, T, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
-                s[S_FTV * n3 + L + n2 * k] += kT * d[k] * hi[k];
+                s[S_C * n3 + L + n2 * k] = kT * d[k] * hi[k];
             dline<n>(D2, S, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
-                s[S_FSV * n3 + L + n2 * k] += kS * d[k] * hi[k];
+                s[S_D * n3 + L + n2 * k] = kS * d[k] * hi[k];
         }
         __syncthreads();
 
-        if (NOPS == 2 && act) {
-            const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
-            const bool two = NOPS == 2 && !(EPI == EPI_LSRK && epi.acc_mode == 1);
-            const View<const TS> V2 =
-                (EPI == EPI_COMB) ? epi.X : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
-            const uint32_t q1 = lane_off(V1, e, L);
-#pragma unroll
-            for (int f = 0; f < 5; ++f)
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                    o1[f][i] = ldo<TS, TC>(V1.p + f * V1.fs + i * V1.is, q1);
-            if (two) {
-                const uint32_t q2 = lane_off(V2, e, L);
-#pragma unroll
-                for (int f = 0; f < 5; ++f)
-#pragma unroll
-                    for (int i = 0; i < n; ++i)
-                        o2[f][i] = ldo<TS, TC>(V2.p + f * V2.fs + i * V2.is, q2);
-            }
-        }
-
-        // ---------------- phase 4: xi-line (j, k) again; write ----------------
+        // ---------------- P4b: xi-line: free surface and tracers done ----------------
         if (act) {
-            const TC *s = &sm[le][0][0];
             const int j0 = L % n, k0 = L / n;
             // eta tendency (:256-272): -(D_x2d HU_bar + D_y2d HV_bar), where the
             // 2D "x" index is j and the 2D "y" index is k.
             double s1 = 0.0, s2 = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                s1 += ops.Dc[j0 * n + m] * bar[le][0][m + n * k0];
-                s2 += ops.Dc[k0 * n + m] * bar[le][1][j0 + n * m];
+                s1 += ops.Dc[j0 * n + m] * bar[par][le][0][m + n * k0];
+                s2 += ops.Dc[k0 * n + m] * bar[par][le][1][j0 + n * m];
             }
             const TC deta = (TC)(-(s1 + s2));
-            TC kv[5][n];
+            TC kv[3][n];
 #pragma unroll
             for (int i = 0; i < n; ++i) {
-                const int nd_ = L * n + i;
-                kv[0][i] = pHu[i] + s[S_FUV * n3 + nd_];
-                kv[1][i] = pHv[i] + s[S_FVV * n3 + nd_];
-                kv[2][i] = deta;
-                kv[3][i] = pHT[i] + s[S_FTV * n3 + nd_];
-                kv[4][i] = pHS[i] + s[S_FSV * n3 + nd_];
+                const int m = L * n + i;
+                kv[0][i] = deta;
+                kv[1][i] = (pHT[i] + s[S_A * n3 + m]) + s[S_C * n3 + m];
+                kv[2][i] = (pHS[i] + s[S_B * n3 + m]) + s[S_D * n3 + m];
             }
-            const TC c0 = (TC)epi.c0, c1 = (TC)epi.c1, c2 = (TC)epi.c2, c3 = (TC)epi.c3;
-            if (EPI == EPI_RHS) {
-                const uint32_t ro = lane_off(epi.R, e, L);
-#pragma unroll
-                for (int f = 0; f < 5; ++f)
-#pragma unroll
-                    for (int i = 0; i < n; ++i)
-                        sto<TS>(epi.R.p + f * epi.R.fs + i * epi.R.is, ro, (TS)kv[f][i]);
-            } else {
-                const uint32_t yo = lane_off(epi.Y, e, L);
-                const uint32_t ao = epi.A.p ? lane_off(epi.A, e, L) : 0u;
-#pragma unroll
-                for (int f = 0; f < 5; ++f)
-#pragma unroll
-                    for (int i = 0; i < n; ++i) {
-                        const TC k = kv[f][i];
-                        TC y;
-                        if (EPI == EPI_STAGE) { // Y = U0 + c1 k ; A = k (acc_mode 1)
-                            if (epi.acc_mode == 1)
-                                sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)k);
-                            y = o1[f][i] + c1 * k;
-                        } else if (EPI == EPI_STAGE_ACC) { // A = A + c2 k ; Y = U0 + c1 k
-                            sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)(o2[f][i] + c2 * k));
-                            y = o1[f][i] + c1 * k;
-                        } else if (EPI == EPI_RK4FIN) { // Y = U0 + c1 (A + k)
-                            y = o1[f][i] + c1 * (o2[f][i] + k);
-                        } else if (EPI == EPI_COMB) { // Y = c0 U0 + c1 (X + c2 k)
-                            y = c0 * o1[f][i] + c1 * (o2[f][i] + c2 * k);
-                        } else { // EPI_LSRK: A = c2 A + c1 k ; Y = X + c3 A
-                            const TC a = (epi.acc_mode == 1) ? c1 * k : c2 * o2[f][i] + c1 * k;
-                            sto<TS>(epi.A.p + f * epi.A.fs + i * epi.A.is, ao, (TS)a);
-                            y = o1[f][i] + c3 * a;
-                        }
-                        sto<TS>(epi.Y.p + f * epi.Y.fs + i * epi.Y.is, yo, (TS)y);
-                    }
-            }
+            write_fields<n, 2, 3, TS, TC, EPI>(epi, e, L, kv);
         }
-        // the next batch rewrites the LDS this batch's phase 4 reads
-        __syncthreads();
+        // No barrier here: this phase reads only its own line's slots and the
+        // column integrals of this batch's parity; the next batch's P1 writes
+        // its own line's slots and the other parity.
     }
 }
 

@@ -139,6 +139,13 @@ class PrimitiveEquationsSolver:
         check(self._l.drf_rk_step_dev(self._ctx, scheme, t, dt, _ptr(U), layout, int(fused),
                                       _sptr(stream)))
 
+    def capture_rk_steps(self, scheme, dt, U, layout=_lib.LAYOUT_SOA, fused=True, n_steps=1):
+        """StepGraph of n_steps drf_rk_step_dev calls on U (in place)."""
+        h = C.c_void_p()
+        check(self._l.drf_graph_rk(self._ctx, scheme, dt, _ptr(U), layout, int(fused), n_steps,
+                                   C.byref(h)))
+        return StepGraph(h)
+
     def max_wave_speed(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         c = C.c_double()
@@ -221,6 +228,26 @@ class AdaptiveTimeController:
                                           self.dt_min, self.dt_max)
 
 
+class StepGraph:
+    """A HIP graph of repeated steps (drf_graph_rk / drf_graph_mode_split):
+    launch() replays them on a stream."""
+
+    def __init__(self, handle):
+        self._l = _lib.load()
+        self._g = handle
+
+    def launch(self, stream=None):
+        check(self._l.drf_graph_launch(self._g, _sptr(stream)))
+
+    def close(self):
+        if getattr(self, "_g", None):
+            self._l.drf_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        self.close()
+
+
 class ModeSplittingSolver:
     """ModeSplittingSolver (include/physics/mode_splitting.hpp:203-266)
     acting on the flat full state [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d]."""
@@ -238,6 +265,13 @@ class ModeSplittingSolver:
         full = np.array(full_state, dtype=np.float64, copy=True)
         check(self._s._l.drf_mode_split_step(self._s.ctx, dt_3d, self.subcycles, dptr(full)))
         return full
+
+    def capture_steps_dev(self, dt_3d, full_state, n_steps):
+        """StepGraph of n_steps step_dev calls on the device full state."""
+        h = C.c_void_p()
+        check(self._s._l.drf_graph_mode_split(self._s.ctx, dt_3d, self.subcycles,
+                                              _ptr(full_state), n_steps, C.byref(h)))
+        return StepGraph(h)
 
     def step_dev(self, dt_3d, full_state, stream=None):
         check(self._s._l.drf_mode_split_step_dev(self._s.ctx, dt_3d, self.subcycles,

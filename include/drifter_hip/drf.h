@@ -156,6 +156,27 @@ drf_status drf_mode_split_step_dev(drf_ctx *ctx, double dt_3d, int n_sub, void *
                                    void *stream);
 
 /* ------------------------------------------------------------------ */
+/* HIP graphs of repeated steps (no counterpart in the reference: the   */
+/* driver calls step() in a loop, simulation_driver.cpp:108-151)       */
+/* ------------------------------------------------------------------ */
+
+typedef struct drf_graph drf_graph;
+/* Captures n_steps calls of drf_rk_step_dev(ctx, scheme, ., dt, U, layout,
+ * fused) on the same state into one HIP graph.  The graph owns the
+ * workspace its kernels use; it keeps U's address and the context's
+ * coefficients, so both must outlive it.  Nothing is executed here. */
+drf_status drf_graph_rk(drf_ctx *ctx, int scheme, double dt, void *U, int layout, int fused,
+                        int n_steps, drf_graph **out);
+/* Same for n_steps mode-split steps (drf_mode_split_step_dev): with the
+ * barotropic subcycles inside each step's kernel, one launch replays
+ * n_steps x n_sub substeps. */
+drf_status drf_graph_mode_split(drf_ctx *ctx, double dt_3d, int n_sub, void *full, int n_steps,
+                                drf_graph **out);
+/* Replays the captured steps on `stream` (asynchronous). */
+drf_status drf_graph_launch(drf_graph *g, void *stream);
+void drf_graph_destroy(drf_graph *g);
+
+/* ------------------------------------------------------------------ */
 /* Octree mesh and face connectivity (OctreeAdapter,                   */
 /* include/mesh/octree_adapter.hpp:103-273)                            */
 /* ------------------------------------------------------------------ */

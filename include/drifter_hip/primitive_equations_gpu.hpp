@@ -5,6 +5,9 @@
 //   RHSFunction / TimeStepper include/solver/time_stepper.hpp:10-33
 //   ModeSplittingSolver       include/physics/mode_splitting.hpp:203-266
 //   AdaptiveTimeController    include/solver/time_stepper.hpp:64-81
+//   MortarSpace               include/dg/mortar.hpp:32-136
+//   ConformingInterface       include/dg/mortar.hpp:187-215
+//   GradientErrorEstimator    include/amr/refinement.hpp, src/amr/refinement.cpp:39-161
 // Vectors are std::vector<double> (Eigen's VecX maps onto them through
 // .data()); failures throw std::runtime_error carrying drf_last_error(), as
 // the reference reports errors by exception.
@@ -188,6 +191,93 @@ class ModeSplittingSolver {
   private:
     const PrimitiveEquationsSolver &solver_;
     int subcycles_;
+};
+
+// MortarSpace (include/dg/mortar.hpp:32-136) for one connection type
+// (DRF_CONN_*).  device < 0 keeps the operators on the host only.
+class MortarSpace {
+  public:
+    MortarSpace(int order, int conn_type, int device = 0)
+    {
+        drf_mortar *m = nullptr;
+        check(drf_mortar_create(order, conn_type, device, &m));
+        m_.reset(m);
+    }
+    int num_dofs() const { return drf_mortar_num_dofs(m_.get()); }
+    int num_fine_elements() const { return drf_mortar_num_fine(m_.get()); }
+    // row-major num_dofs x num_dofs matrices; fine operators stacked
+    Vec mass_matrix() const { return op(0); }
+    Vec mass_matrix_inv() const { return op(1); }
+    Vec projection_coarse() const { return op(2); }
+    Vec lift_coarse() const { return op(3); }
+    Vec projection_fine() const { return op(4); }
+    Vec lift_fine() const { return op(5); }
+    // compute_mortar_flux (mortar.hpp:103) for n interfaces, device buffers
+    void compute_mortar_flux(int flux_kind, const double *params, int64_t n, const void *Uc,
+                             const void *Uf, const void *normals, void *rhs_c, void *rhs_f,
+                             void *stream = nullptr)
+    {
+        check(drf_mortar_flux_batch(m_.get(), flux_kind, params, n, Uc, Uf, normals, rhs_c,
+                                    rhs_f, stream));
+    }
+
+  private:
+    Vec op(int which) const
+    {
+        const size_t mm = (size_t)num_dofs() * num_dofs();
+        Vec v((which >= 4 ? num_fine_elements() : 1) * mm);
+        double *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        p[which] = v.data();
+        check(drf_mortar_operators(m_.get(), p[0], p[1], p[2], p[3], p[4], p[5]));
+        return v;
+    }
+    struct Del {
+        void operator()(drf_mortar *m) const { drf_mortar_destroy(m); }
+    };
+    std::unique_ptr<drf_mortar, Del> m_;
+};
+
+// ConformingInterface::compute_flux (include/dg/mortar.hpp:202) for n faces
+struct ConformingInterface {
+    int order;
+    void compute_flux(int flux_kind, const double *params, int64_t n, const void *UL,
+                      const void *UR, const void *normals, void *rhs_l, void *rhs_r,
+                      void *stream = nullptr) const
+    {
+        check(drf_conforming_flux_batch(order, flux_kind, params, n, UL, UR, normals, rhs_l,
+                                        rhs_r, stream));
+    }
+};
+
+// GradientErrorEstimator(basis, params) + estimate + the marking of
+// AdaptiveMeshRefinement::mark_elements, on device buffers.
+class GradientErrorEstimator {
+  public:
+    GradientErrorEstimator(int order, Real refine_threshold = 0.1, Real coarsen_threshold = 0.01,
+                           Real smoothness_factor = 0.5, int max_level = 8, int min_level = 0,
+                           int device = 0)
+        : order_(order)
+    {
+        drf_estimator *e = nullptr;
+        check(drf_estimator_create(order, refine_threshold, coarsen_threshold, smoothness_factor,
+                                   max_level, min_level, device, &e));
+        e_.reset(e);
+    }
+    void estimate(int64_t n_elem, const void *field, const void *h_char, const void *levels,
+                  void *value, void *mask, void *refine, void *coarsen, void *action,
+                  void *mask_out, void *stream = nullptr)
+    {
+        const int64_t nd = (int64_t)(order_ + 1) * (order_ + 1) * (order_ + 1);
+        check(drf_estimate_dev(e_.get(), n_elem, field, nd, h_char, levels, value, mask, refine,
+                               coarsen, action, mask_out, nullptr, stream));
+    }
+
+  private:
+    int order_;
+    struct Del {
+        void operator()(drf_estimator *e) const { drf_estimator_destroy(e); }
+    };
+    std::unique_ptr<drf_estimator, Del> e_;
 };
 
 } // namespace drifter_gpu

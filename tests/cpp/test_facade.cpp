@@ -119,6 +119,27 @@ int main()
             if (q < 6 || !(r <= 1e-12))
                 std::printf("mode split field %d rel L2 %.3e\n", q, r);
         }
+        // MortarSpace operators (host copies) against the oracle's
+        for (int t = DRF_CONN_SAME_LEVEL; t <= DRF_CONN_FINE_2X2; ++t) {
+            drifter_gpu::MortarSpace m(order, t, -1);
+            const int nm = m.num_dofs(), nf = m.num_fine_elements();
+            Vec M(nm * nm), Mi(nm * nm), Pc(nm * nm), Lc(nm * nm), Pf(4 * nm * nm),
+                Lf(4 * nm * nm);
+            const int onf = orc_mortar_operators(order, t, M.data(), Mi.data(), Pc.data(),
+                                                 Lc.data(), Pf.data(), Lf.data());
+            Vec gM = m.mass_matrix(), gPf = m.projection_fine(), gLf = m.lift_fine();
+            double dM = 0, dP = 0, dL = 0;
+            for (int i = 0; i < nm * nm; ++i)
+                dM = std::fmax(dM, std::fabs(gM[i] - M[i]));
+            for (int i = 0; i < nf * nm * nm; ++i) {
+                dP = std::fmax(dP, std::fabs(gPf[i] - Pf[i]));
+                dL = std::fmax(dL, std::fabs(gLf[i] - Lf[i]));
+            }
+            const bool ok = onf == nf && dM == 0.0 && dL == 0.0 && dP < 1e-12;
+            fails += !ok;
+            std::printf("mortar type %d: %d fine, |dM| %.1e |dPf| %.1e |dLf| %.1e %s\n", t, nf,
+                        dM, dP, dL, ok ? "ok" : "MISMATCH");
+        }
     } catch (const std::exception &ex) {
         std::printf("FAIL: exception %s\n", ex.what());
         return 1;
