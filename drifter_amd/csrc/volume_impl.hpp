@@ -8,8 +8,9 @@
 // derivatives of DG3DElementOperator::{gradient,divergence}_reference
 // (src/dg/operators_3d.cpp:58-79) applied by sum factorisation instead of
 // dense Kronecker mat-vecs (D_xi = D (x) I (x) I, src/dg/basis_hexahedron.cpp:321-357).
-// Second derivatives D(D u) are applied as (D D) u with D^2 formed once on the
-// host: 18 one-dimensional contractions per node instead of 26.
+// Second derivatives are applied as D (D u) in FP64, two line contractions
+// as the reference applies its Kronecker gradient twice (only D then sits in
+// SGPRs), and as (D D) u with D^2 formed once on the host in FP32.
 //
 // Work decomposition.  A workgroup owns EPB elements per batch; every element
 // is handled by n^2 threads, each owning one grid line of n nodes.  Line
@@ -60,7 +61,7 @@ template <int N> struct Cfg {
 // Operators sized for one order, passed by value (kernel-argument SGPRs).
 template <int n, class TC> struct KOps {
     TC D[n * n];   // 1D derivative matrix, row-major
-    TC D2[n * n];  // D * D (second derivative)
+    TC D2[n * n];  // D * D (second derivative; read by the FP32 kernel only)
     double Dc[n * n]; // FP64 D for the free-surface column integrals
     double sw[n];     // 0.5 * LGL weights
 };
@@ -102,6 +103,23 @@ __device__ __forceinline__ void dline(const TC *__restrict__ M, const TC (&x)[n]
         for (int m = 1; m < n; ++m)
             s += M[i * n + m] * x[m];
         y[i] = s;
+    }
+}
+
+// Second derivative along one line.  FP64: D (D x), two contractions as the
+// reference applies its gradient twice, with only D in SGPRs.  FP32: D^2
+// formed once on the host (one contraction; two float matrices fit the SGPR
+// file).
+template <int n, class TC>
+__device__ __forceinline__ void d2line(const TC *__restrict__ D, const TC *__restrict__ D2,
+                                       const TC (&x)[n], TC (&y)[n])
+{
+    if constexpr (sizeof(TC) == 8) {
+        TC t[n];
+        dline<n>(D, x, t);
+        dline<n>(D, t, y);
+    } else {
+        dline<n>(D2, x, y);
     }
 }
 
@@ -225,6 +243,10 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
     const TC g = (TC)ph.g, nu_h = (TC)ph.nu_h, nu_v = (TC)ph.nu_v, kT = (TC)ph.kT, kS = (TC)ph.kS;
     const TC *__restrict__ D = ops.D;
     const TC *__restrict__ D2 = ops.D2;
+    // the FP64 D of the column integrals is D itself when TC is double:
+    // reading it from D keeps one operator fewer in SGPRs
+    const double *__restrict__ Dc =
+        std::is_same<TC, double>::value ? (const double *)ops.D : ops.Dc;
 
     // XCD-aware batch order.  Workgroups are dealt round-robin to the 8 XCDs
     // (blockIdx % 8), each with its own L2.  Renumbering them so that
@@ -303,11 +325,11 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
 #pragma unroll
             for (int i = 0; i < n; ++i)
                 pHu[i] += (-g * H[i]) * d[i];
-            dline<n>(D2, u, d);
+            d2line<n>(D, D2, u, d);
 #pragma unroll
             for (int i = 0; i < n; ++i)
                 pHu[i] += H[i] * (nu_h * d[i]);
-            dline<n>(D2, v, d);
+            d2line<n>(D, D2, v, d);
 #pragma unroll
             for (int i = 0; i < n; ++i)
                 pHv[i] += H[i] * (nu_h * d[i]);
@@ -361,7 +383,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
 #pragma unroll
             for (int j = 0; j < n; ++j)
                 aHu[j] = -d[j];
-            dline<n>(D2, u, d);
+            d2line<n>(D, D2, u, d);
 #pragma unroll
             for (int j = 0; j < n; ++j)
                 aHu[j] += H[j] * (nu_h * d[j]);
@@ -373,7 +395,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
 #pragma unroll
             for (int j = 0; j < n; ++j)
                 aHv[j] += (-g * H[j]) * d[j];
-            dline<n>(D2, v, d);
+            d2line<n>(D, D2, v, d);
 #pragma unroll
             for (int j = 0; j < n; ++j)
                 aHv[j] += H[j] * (nu_h * d[j]);
@@ -398,11 +420,11 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             }
             // vertical_diffusion :357-375: nu_v * d2/dzeta2 * Hinv
             TC d[n];
-            dline<n>(D2, u, d);
+            d2line<n>(D, D2, u, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
                 s[S_A * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
-            dline<n>(D2, v, d);
+            d2line<n>(D, D2, v, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
                 s[S_B * n3 + L + n2 * k] += nu_v * d[k] * hi[k];
@@ -486,11 +508,11 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 hi[k] = hinv_of(s[S_F * n3 + m]);
             }
             TC d[n];
-            dline<n>(D2, T, d);
+            d2line<n>(D, D2, T, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
                 s[S_C * n3 + L + n2 * k] = kT * d[k] * hi[k];
-            dline<n>(D2, S, d);
+            d2line<n>(D, D2, S, d);
 #pragma unroll
             for (int k = 0; k < n; ++k)
                 s[S_D * n3 + L + n2 * k] = kS * d[k] * hi[k];
@@ -505,8 +527,8 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             double s1 = 0.0, s2 = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                s1 += ops.Dc[j0 * n + m] * bar[par][le][0][m + n * k0];
-                s2 += ops.Dc[k0 * n + m] * bar[par][le][1][j0 + n * m];
+                s1 += Dc[j0 * n + m] * bar[par][le][0][m + n * k0];
+                s2 += Dc[k0 * n + m] * bar[par][le][1][j0 + n * m];
             }
             const TC deta = (TC)(-(s1 + s2));
             TC kv[3][n];
