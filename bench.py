@@ -201,6 +201,27 @@ def measure_coastline(args, stream):
     return out
 
 
+def measure_mode_split(case, stream):
+    """Mode splitting on the 64x64x8, N=4 state: 10 baroclinic steps with 20
+    barotropic substeps each, launched step by step and as one HIP graph."""
+    import torch
+
+    from drifter_amd.solver import CoriolisParameter, ModeSplittingSolver
+
+    ms = ModeSplittingSolver(4, subcycles=20)
+    ms.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    full = torch.from_numpy(case.full_state()).to("cuda")
+    dt = 1e-3
+    work = full.clone()
+    eager = time_launches(lambda: [ms.step_dev(dt, work, stream) for _ in range(10)], 3, 1, stream)
+    work = full.clone()
+    g = ms.capture_steps_dev(dt, work, 10)
+    graph = time_launches(lambda: g.launch(stream), 3, 1, stream)
+    g.close()
+    log(f"mode split, 10 steps x 20 substeps: eager {eager:.3f} ms, graph {graph:.3f} ms")
+    return {"mode_split_10x20_eager_ms": round(eager, 4), "mode_split_10x20_graph_ms": round(graph, 4)}
+
+
 def measure_indicator(stream):
     """Learned indicator: training on 50 seeded coastline states (about 2M
     element/label pairs), misclassification against the estimator on 5
@@ -318,6 +339,7 @@ def main():
     extras = None
     if not args.rhs_only:
         extras = measure_extras(args, case, s64, U, r64, stream, tune)
+        extras.update(measure_mode_split(case, stream))
         extras.update(measure_coastline(args, stream))
         extras.update(measure_indicator(stream))
 

@@ -57,8 +57,10 @@ __device__ __forceinline__ float gelu_d(float z)
 // acc += A[m0:m0+16][0:K] * B[0:K][n0:n0+16]; A(m,k) = a[m*am + k*ak],
 // B(k,n) = b[k*bk + n*bn], BF16 in LDS.  Lane l supplies A rows m0 + l%16
 // and B columns n0 + l%16 at k = 4(l/16) .. 4(l/16)+3 of each 16-wide step;
-// it receives C rows m0 + 4(l/16) + t, column n0 + l%16, in acc[t].
-template <int K>
+// it receives C rows m0 + 4(l/16) + t, column n0 + l%16, in acc[t].  When
+// an operand's k index is contiguous (ak == 1 / bk == 1: ACONT / BCONT) its
+// four values are one 8-byte LDS read (row pitches are multiples of 8 bytes).
+template <int K, bool ACONT, bool BCONT>
 __device__ __forceinline__ f32x4 mfma_block(const uint16_t *a, int am, int ak, const uint16_t *b,
                                             int bk, int bn, int m0, int n0, f32x4 acc)
 {
@@ -67,17 +69,26 @@ __device__ __forceinline__ f32x4 mfma_block(const uint16_t *a, int am, int ak, c
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 16) {
         bf16x4 av, bv;
+        if (ACONT) {
+            av = *reinterpret_cast<const bf16x4 *>(a + (m0 + r) * am + k0 + kq);
+        } else {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            av[t] = (short)a[(m0 + r) * am + (k0 + kq + t) * ak];
-            bv[t] = (short)b[(k0 + kq + t) * bk + (n0 + r) * bn];
+            for (int t = 0; t < 4; ++t)
+                av[t] = (short)a[(m0 + r) * am + (k0 + kq + t) * ak];
+        }
+        if (BCONT) {
+            bv = *reinterpret_cast<const bf16x4 *>(b + (n0 + r) * bn + k0 + kq);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                bv[t] = (short)b[(k0 + kq + t) * bk + (n0 + r) * bn];
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
     return acc;
 }
 
-template <int R, bool TRAIN> struct Smem {
+template <int R, bool TRAIN> struct alignas(16) Smem {
     uint16_t x[R][LX];
     uint16_t w1[NH][LX];
     uint16_t w2[NH][LH];
@@ -137,7 +148,7 @@ __device__ void forward_tile(Smem<R, TRAIN> &s, float b3)
     for (int blk = w; blk < NB; blk += 4) {
         const int m0 = (blk / (NH / 16)) * 16, n0 = (blk % (NH / 16)) * 16;
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma_block<NI>(&s.x[0][0], LX, 1, &s.w1[0][0], 1, LX, m0, n0, acc);
+        acc = mfma_block<NI, true, true>(&s.x[0][0], LX, 1, &s.w1[0][0], 1, LX, m0, n0, acc);
         const int col = n0 + (lane & 15);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -152,7 +163,7 @@ __device__ void forward_tile(Smem<R, TRAIN> &s, float b3)
     for (int blk = w; blk < NB; blk += 4) {
         const int m0 = (blk / (NH / 16)) * 16, n0 = (blk % (NH / 16)) * 16;
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma_block<NH>(&s.a1[0][0], LH, 1, &s.w2[0][0], 1, LH, m0, n0, acc);
+        acc = mfma_block<NH, true, true>(&s.a1[0][0], LH, 1, &s.w2[0][0], 1, LH, m0, n0, acc);
         const int col = n0 + (lane & 15);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -179,6 +190,8 @@ __device__ void forward_tile(Smem<R, TRAIN> &s, float b3)
     __syncthreads();
 }
 
+// Persistent inference: weights staged once per workgroup; the next tile's
+// rows are loaded into registers while the current tile is computed.
 template <int R>
 __global__ void __launch_bounds__(256)
     mlp_forward_kernel(int64_t n, const uint16_t *__restrict__ X, const float *__restrict__ w,
@@ -188,10 +201,39 @@ __global__ void __launch_bounds__(256)
     stage_weights(s, w, wq);
     const float b3 = w[OB3];
     const int64_t ntiles = (n + R - 1) / R;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    constexpr int VPR = NI / 8;                 // 16-byte vectors per row
+    constexpr int VPT = (R * VPR + 255) / 256;  // vectors per thread per tile
+    uint4 nxt[VPT];
+    auto fetch = [&](int64_t r0) {
+#pragma unroll
+        for (int q = 0; q < VPT; ++q) {
+            const int v = threadIdx.x + q * 256;
+            const int r = v / VPR, c = (v % VPR) * 8;
+            nxt[q] = make_uint4(0, 0, 0, 0);
+            if (v < R * VPR && r0 + r < n)
+                nxt[q] = *reinterpret_cast<const uint4 *>(X + (r0 + r) * NI + c);
+        }
+    };
+    int64_t tile = blockIdx.x;
+    if (tile < ntiles)
+        fetch(tile * R);
+    for (; tile < ntiles; tile += gridDim.x) {
         const int64_t r0 = tile * R;
-        __syncthreads(); // previous tile's pred read, weights staged
-        load_tile(s, X, n, r0);
+        __syncthreads(); // previous tile's rows and pred consumed, weights staged
+#pragma unroll
+        for (int q = 0; q < VPT; ++q) {
+            const int v = threadIdx.x + q * 256;
+            if (v < R * VPR) {
+                const int r = v / VPR, c = (v % VPR) * 8;
+                uint32_t *dst = reinterpret_cast<uint32_t *>(&s.x[r][c]);
+                dst[0] = nxt[q].x;
+                dst[1] = nxt[q].y;
+                dst[2] = nxt[q].z;
+                dst[3] = nxt[q].w;
+            }
+        }
+        if (tile + gridDim.x < ntiles)
+            fetch((tile + gridDim.x) * R);
         __syncthreads();
         forward_tile(s, b3);
         for (int r = threadIdx.x; r < R; r += blockDim.x)
@@ -255,7 +297,7 @@ __global__ void __launch_bounds__(256)
         for (int blk = w_; blk < NB; blk += 4) {
             const int m0 = (blk / (NH / 16)) * 16, n0 = (blk % (NH / 16)) * 16;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma_block<NH>(&s.d2[0][0], LH, 1, &s.w2[0][0], LH, 1, m0, n0, acc);
+            acc = mfma_block<NH, true, false>(&s.d2[0][0], LH, 1, &s.w2[0][0], LH, 1, m0, n0, acc);
             const int c = n0 + (lane & 15);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -273,14 +315,16 @@ __global__ void __launch_bounds__(256)
         for (int q = 0; q < 4; ++q) {
             const int blk = w_ * 4 + q;
             const int m0 = (blk / 4) * 16, n0 = (blk % 4) * 16;
-            gw2[q] = mfma_block<R>(&s.d2[0][0], 1, LH, &s.a1[0][0], LH, 1, m0, n0, gw2[q]);
+            gw2[q] = mfma_block<R, false, false>(&s.d2[0][0], 1, LH, &s.a1[0][0], LH, 1, m0, n0,
+                                                 gw2[q]);
         }
         // dW1[o][i] += sum_r d1[r][o] x[r][i]: 32 blocks, 8 per wavefront
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int blk = w_ * 8 + q;
             const int m0 = (blk / 8) * 16, n0 = (blk % 8) * 16;
-            gw1[q] = mfma_block<R>(&s.d1[0][0], 1, LH, &s.x[0][0], LX, 1, m0, n0, gw1[q]);
+            gw1[q] = mfma_block<R, false, false>(&s.d1[0][0], 1, LH, &s.x[0][0], LX, 1, m0, n0,
+                                                 gw1[q]);
         }
     }
     // write this workgroup's partial gradient
@@ -389,7 +433,7 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-constexpr int kRowsFwd = 64, kRowsTrain = 32;
+constexpr int kRowsFwd = 32, kRowsTrain = 32;
 
 } // namespace
 
@@ -405,7 +449,21 @@ hipError_t launch_mlp_forward(int64_t n, const uint16_t *X, const float *w, cons
     if (n <= 0)
         return hipSuccess;
     const int64_t tiles = (n + kRowsFwd - 1) / kRowsFwd;
-    const unsigned grid = (unsigned)(tiles < 2048 ? tiles : 2048);
+    // persistent: a few resident workgroups per CU (LDS-bound), weights
+    // staged once per workgroup
+    static const int cap = [] {
+        int dev = 0, per = 1;
+        hipDeviceProp_t p;
+        int cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            cus = p.multiProcessorCount;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mlp_forward_kernel<kRowsFwd>, 256,
+                                                         0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        return cus * per;
+    }();
+    const unsigned grid = (unsigned)(tiles < cap ? tiles : cap);
     hipLaunchKernelGGL(mlp_forward_kernel<kRowsFwd>, dim3(grid), dim3(256), 0, s, n, X, w, wq,
                        pred);
     return hipGetLastError();
