@@ -38,7 +38,7 @@ namespace drf {
 namespace vol {
 
 // LDS slots per element node, used in two rounds (see the kernel).
-enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_NF };
+enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_G, S_NF };
 
 template <int N> struct Cfg {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -206,17 +206,18 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
     }
 }
 
-// The kernel.  Six LDS slots per element node, reused in two rounds
-// (momentum, then tracers), so a workgroup of 5 N=4 elements needs 30 KB of
-// staging plus 4 KB of column integrals, and the CU's occupancy is set by
-// registers (4 workgroups at N=4 FP64) rather than by LDS:
+// The kernel.  Seven LDS slots per element node, six of them reused in two
+// rounds (momentum, then tracers), so a workgroup of 5 N=4 elements needs
+// 35 KB of staging plus 4 KB of column integrals, and the CU's occupancy is
+// set by registers (4 workgroups at N=4 FP64) rather than by LDS:
 //
 //   P1  xi-owner   xi-derivatives of the momentum terms (registers), column
-//                  integrals (FP64), stages A=Hu v, B=Hv v, C=eta, D=u, E=v, F=H
+//                  integrals (FP64), stages A=Hu v, B=Hv v, C=eta, D=u, E=v, F=H,
+//                  G=1/H (the dry-node guard of update_derived applied once)
 //   P2a eta-owner  eta-derivatives of the momentum terms -> A, B
 //   P3a zeta-owner vertical diffusion of u, v             -> A, B +=
 //   P4a xi-owner   dHu, dHv complete (written out); xi-derivatives of the
-//                  tracer terms; restages A=v HT, B=v HS, C=T, D=S (F=H kept)
+//                  tracer terms; restages A=v HT, B=v HS, C=T, D=S (F, G kept)
 //   P23b eta-owner eta-derivatives of the tracer fluxes    -> A, B
 //        zeta-owner vertical diffusion of T, S (same phase) -> C, D
 //   P4b xi-owner   dHT, dHS and d eta complete (written out)
@@ -282,11 +283,12 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
 
         // ---------------- P1: xi-line (j, k) = (L % n, L / n) ----------------
         if (act) {
-            TC H[n], u[n], v[n];
+            TC H[n], u[n], v[n], hinv[n];
 #pragma unroll
             for (int i = 0; i < n; ++i) {
                 H[i] = cur.et[i] + cur.hb[i];
-                const TC hi = hinv_of(H[i]);
+                hinv[i] = hinv_of(H[i]);
+                const TC hi = hinv[i];
                 u[i] = cur.hu[i] * hi;
                 v[i] = cur.hv[i] * hi;
                 ht[i] = cur.ht[i];
@@ -342,6 +344,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 s[S_D * n3 + m] = u[i];
                 s[S_E * n3 + m] = v[i];
                 s[S_F * n3 + m] = H[i];
+                s[S_G * n3 + m] = hinv[i];
             }
             // Column integrals of compute_eta_rhs (:244-254): over the xi index,
             // summed in node order from 0, in FP64 at every precision.
@@ -416,7 +419,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 const int m = L + n2 * k;
                 u[k] = s[S_D * n3 + m];
                 v[k] = s[S_E * n3 + m];
-                hi[k] = hinv_of(s[S_F * n3 + m]);
+                hi[k] = s[S_G * n3 + m];
             }
             // vertical_diffusion :357-375: nu_v * d2/dzeta2 * Hinv
             TC d[n];
@@ -448,7 +451,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 const int m = L * n + i;
                 u[i] = s[S_D * n3 + m];
                 v[i] = s[S_E * n3 + m];
-                hi[i] = hinv_of(s[S_F * n3 + m]);
+                hi[i] = s[S_G * n3 + m];
             }
             // tracer_advection :377-393 along xi
             TC x[n], d[n];
@@ -505,7 +508,7 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 const int m = L + n2 * k;
                 T[k] = s[S_C * n3 + m];
                 S[k] = s[S_D * n3 + m];
-                hi[k] = hinv_of(s[S_F * n3 + m]);
+                hi[k] = s[S_G * n3 + m];
             }
             TC d[n];
             d2line<n>(D, D2, T, d);

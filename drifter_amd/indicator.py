@@ -107,7 +107,8 @@ def build_dataset(fields, seeds):
 
 
 def train(model, X, y, epochs=4, batch=8192, lr=2e-3, seed=0):
-    """Adam over shuffled minibatches; returns the last epoch's mean loss."""
+    """Adam over shuffled minibatches with a cosine-decayed learning rate;
+    returns the last epoch's mean loss."""
     import torch
 
     n = X.shape[0]
@@ -115,12 +116,13 @@ def train(model, X, y, epochs=4, batch=8192, lr=2e-3, seed=0):
     loss = torch.zeros(1, dtype=torch.float32, device=X.device)
     total = 0.0
     for ep in range(epochs):
+        lr_ep = lr * 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
         perm = torch.randperm(n, generator=g).to(X.device)
         Xs, ys = X[perm].contiguous(), y[perm].contiguous()
         acc = torch.zeros(1, dtype=torch.float64, device=X.device)
         nb = 0
         for b0 in range(0, n - batch + 1, batch):
-            model.train_step(Xs[b0:b0 + batch], ys[b0:b0 + batch], lr, loss=loss)
+            model.train_step(Xs[b0:b0 + batch], ys[b0:b0 + batch], lr_ep, loss=loss)
             acc += loss
             nb += 1
         total = float(acc.cpu()[0]) / max(nb, 1)
