@@ -222,6 +222,55 @@ def measure_mode_split(case, stream):
     return {"mode_split_10x20_eager_ms": round(eager, 4), "mode_split_10x20_graph_ms": round(graph, 4)}
 
 
+def measure_faces(stream):
+    """Face components at coastline-mesh scale, N = 3: pointwise fluxes,
+    conforming faces, 2:1 mortars (2x2 sub-faces) and face-trace gathers."""
+    import numpy as np
+    import torch
+
+    from drifter_amd import interface as itf
+
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(1)
+    out = {}
+    npts = 1 << 22
+    UL = (torch.rand((3, npts), device=dev, dtype=torch.float64, generator=g) + 0.5)
+    UR = (torch.rand((3, npts), device=dev, dtype=torch.float64, generator=g) + 0.5)
+    nrm = torch.zeros((3, npts), device=dev, dtype=torch.float64)
+    nrm[0] = 0.6
+    nrm[1] = 0.8
+    ms = time_launches(lambda: itf.flux_points(itf.SW_HLLC, UL, UR, nrm, [9.81], stream=stream),
+                       20, 3, stream)
+    out["flux_hllc_4M_points_us"] = round(ms * 1e3, 1)
+    out["flux_hllc_gb_s"] = round(npts * 8 * 12 / (ms * 1e-3) / 1e9, 1)
+    nf, nfaces = 16, 200000
+    A = torch.rand((nfaces, nf), device=dev, dtype=torch.float64, generator=g)
+    B = torch.rand((nfaces, nf), device=dev, dtype=torch.float64, generator=g)
+    fn = torch.zeros((nfaces, 3), device=dev, dtype=torch.float64)
+    fn[:, 2] = 1.0
+    ci = itf.ConformingInterface(3)
+    ms = time_launches(lambda: ci.compute_flux(itf.UPWIND, [0.3, 0.2, 0.1], A, B, fn, stream),
+                       20, 3, stream)
+    out["conforming_200k_faces_n3_us"] = round(ms * 1e3, 1)
+    m = itf.MortarSpace(3, itf.CONN_FINE_2X2)
+    nif = 100000
+    Uc = torch.rand((nif, nf), device=dev, dtype=torch.float64, generator=g)
+    Uf = torch.rand((nif, 4, nf), device=dev, dtype=torch.float64, generator=g)
+    mn = fn[:nif].contiguous()
+    ms = time_launches(lambda: m.compute_mortar_flux(itf.LF_ADV, [0.3, 0.2, 0.1, 0.5], Uc, Uf, mn,
+                                                     stream), 20, 3, stream)
+    out["mortar_2x2_100k_interfaces_n3_us"] = round(ms * 1e3, 1)
+    E = 39280
+    field = torch.rand((E, 64), device=dev, dtype=torch.float64, generator=g)
+    el = torch.arange(E, device=dev, dtype=torch.int64).repeat_interleave(6)
+    fa = torch.arange(6, device=dev, dtype=torch.int8).repeat(E)
+    ms = time_launches(lambda: itf.gather_face_traces(3, field, el, fa, E, stream=stream), 20, 3,
+                       stream)
+    out["gather_235680_faces_n3_us"] = round(ms * 1e3, 1)
+    log(f"faces: {out}")
+    return out
+
+
 def measure_indicator(stream):
     """Learned indicator: training on 50 seeded coastline states (about 2M
     element/label pairs, 16 epochs), misclassification against the estimator on 5
@@ -341,6 +390,7 @@ def main():
         extras = measure_extras(args, case, s64, U, r64, stream, tune)
         extras.update(measure_mode_split(case, stream))
         extras.update(measure_coastline(args, stream))
+        extras.update(measure_faces(stream))
         extras.update(measure_indicator(stream))
 
     cpu = None if args.no_cpu else cpu_baseline(case)

@@ -68,8 +68,9 @@ __device__ __forceinline__ void wave_sync_lds()
 
 // Pass 1: raw indicator value[e] = (max - min) * h_char[e] and the
 // directional mask; partial[block] = max of the block's values.
+template <int n>
 __global__ void __launch_bounds__(64 * kWaves)
-    estimate_kernel(int n, int64_t E, const double *__restrict__ field, int64_t es,
+    estimate_kernel(int64_t E, const double *__restrict__ field, int64_t es,
                     const double *__restrict__ hchar, double smooth, double *__restrict__ value,
                     uint8_t *__restrict__ mask, double *__restrict__ partial)
 {
@@ -226,9 +227,9 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 // contractions (xi, then eta, then zeta) through the wave's LDS slice, lanes
 // owning grid lines.  OUT = double (coefficients) or uint16_t (BF16
 // features, scaled, mean mode zeroed).
-template <class OUT>
+template <int n, class OUT>
 __global__ void __launch_bounds__(64 * kWaves)
-    modal_kernel(int n, int64_t E, const double *__restrict__ u, int64_t es, ModalOps ops,
+    modal_kernel(int64_t E, const double *__restrict__ u, int64_t es, ModalOps ops,
                  OUT *__restrict__ out, int64_t out_stride, int out_offset, double scale,
                  const double *__restrict__ elem_scale)
 {
@@ -306,8 +307,21 @@ hipError_t launch_estimate(int n, int64_t E, const double *field, int64_t es, co
         return hipSuccess;
     const unsigned g1 = grid_for(n, E, kEstimatorPartials);
     const size_t lds = sizeof(double) * (size_t)kWaves * line_map(n).epw * n * n * n;
-    hipLaunchKernelGGL(estimate_kernel, dim3(g1), dim3(64 * kWaves), lds, s, n, E, field, es, hchar,
-                       p.smoothness_factor, value, mask, partial);
+#define DRF_EST(NN)                                                                           \
+    hipLaunchKernelGGL(estimate_kernel<NN>, dim3(g1), dim3(64 * kWaves), lds, s, E, field, es,  \
+                       hchar, p.smoothness_factor, value, mask, partial)
+    switch (n) {
+    case 2: DRF_EST(2); break;
+    case 3: DRF_EST(3); break;
+    case 4: DRF_EST(4); break;
+    case 5: DRF_EST(5); break;
+    case 6: DRF_EST(6); break;
+    case 7: DRF_EST(7); break;
+    case 8: DRF_EST(8); break;
+    case 9: DRF_EST(9); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef DRF_EST
     hipError_t err = hipGetLastError();
     if (err != hipSuccess)
         return err;
@@ -326,12 +340,27 @@ hipError_t launch_modal(int n, int64_t E, const double *u, int64_t es, const Mod
         return hipSuccess;
     const unsigned g = grid_for(n, E, 8192);
     const size_t lds = sizeof(double) * (size_t)kWaves * 2 * line_map(n).epw * n * n * n;
-    if (coef)
-        hipLaunchKernelGGL(modal_kernel<double>, dim3(g), dim3(64 * kWaves), lds, s, n, E, u, es,
-                           ops, coef, out_stride, out_offset, scale, elem_scale);
-    else
-        hipLaunchKernelGGL(modal_kernel<uint16_t>, dim3(g), dim3(64 * kWaves), lds, s, n, E, u, es,
-                           ops, bf16, out_stride, out_offset, scale, elem_scale);
+#define DRF_MODAL(NN)                                                                         \
+    do {                                                                                      \
+        if (coef)                                                                             \
+            hipLaunchKernelGGL((modal_kernel<NN, double>), dim3(g), dim3(64 * kWaves), lds, s, E, \
+                               u, es, ops, coef, out_stride, out_offset, scale, elem_scale);  \
+        else                                                                                  \
+            hipLaunchKernelGGL((modal_kernel<NN, uint16_t>), dim3(g), dim3(64 * kWaves), lds, s,  \
+                               E, u, es, ops, bf16, out_stride, out_offset, scale, elem_scale); \
+    } while (0)
+    switch (n) {
+    case 2: DRF_MODAL(2); break;
+    case 3: DRF_MODAL(3); break;
+    case 4: DRF_MODAL(4); break;
+    case 5: DRF_MODAL(5); break;
+    case 6: DRF_MODAL(6); break;
+    case 7: DRF_MODAL(7); break;
+    case 8: DRF_MODAL(8); break;
+    case 9: DRF_MODAL(9); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef DRF_MODAL
     return hipGetLastError();
 }
 

@@ -9,10 +9,11 @@
 //   ConformingInterface::compute_flux   src/dg/mortar.cpp:320-343
 //   MortarSpace::compute_mortar_flux    src/dg/mortar.cpp:212-258
 //   LGL face extraction                 src/dg/basis_hexahedron.cpp:431-522
-// Pointwise fluxes run one face point per lane over structure-of-arrays
-// inputs.  Mortars run one workgroup per batch of interfaces of one
-// connection type: the type's projection and lift operators (built once on
-// the host) are staged in LDS and every lane owns one mortar DOF.
+// Pointwise fluxes and conforming faces run one face point per lane.  Mortars
+// run persistent workgroups over the interfaces of one connection type: the
+// type's projection and lift operators (built once on the host) are staged in
+// LDS when they fit, and every lane owns one mortar DOF of one of the
+// workgroup's interfaces.
 #include "drf_internal.hpp"
 #include "interface.hpp"
 
@@ -220,39 +221,41 @@ __global__ void __launch_bounds__(256)
 }
 
 // ConformingInterface::compute_flux: N-point LGL face rule (the quadrature is
-// built with `order` points, quadrature_3d.cpp:13-31), lanes = face points.
-__global__ void __launch_bounds__(128)
+// built with `order` points, quadrature_3d.cpp:13-31).  One face point per
+// lane over the flattened (face, point) range.
+__global__ void __launch_bounds__(256)
     conforming_kernel(int kind, FluxParams p, int nf, FaceRule rule, int64_t nfaces,
                       const double *__restrict__ UL, const double *__restrict__ UR,
                       const double *__restrict__ nrm, double *__restrict__ rl,
                       double *__restrict__ rr)
 {
-    const int64_t face = blockIdx.x;
-    if (face >= nfaces)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfaces * nf)
         return;
+    const int64_t face = t / nf;
+    const int q = (int)(t - face * nf);
     const int nq1 = rule.nq1;
-    const double n[3] = {nrm[3 * face], nrm[3 * face + 1], nrm[3 * face + 2]};
-    for (int q = threadIdx.x; q < nf; q += blockDim.x) {
-        double l = 0.0, r = 0.0;
-        if (q < nq1 * nq1) {
-            const int i = q % nq1, j = q / nq1;
-            const double w = rule.w[i] * rule.w[j];
-            const double F = scalar_flux(kind, UL[face * nf + q], UR[face * nf + q], n, p);
-            l = 0.0 + w * F;
-            r = 0.0 - w * F;
-        }
-        rl[face * nf + q] = l;
-        rr[face * nf + q] = r;
+    double l = 0.0, r = 0.0;
+    if (q < nq1 * nq1) {
+        const double n[3] = {nrm[3 * face], nrm[3 * face + 1], nrm[3 * face + 2]};
+        const int i = q % nq1, j = q / nq1;
+        const double w = rule.w[i] * rule.w[j];
+        const double F = scalar_flux(kind, UL[t], UR[t], n, p);
+        l = 0.0 + w * F;
+        r = 0.0 - w * F;
     }
+    rl[t] = l;
+    rr[t] = r;
 }
 
 // MortarSpace::compute_mortar_flux for a batch of interfaces of one type.
 // ops = [Minv | Pc | Lc | Pf[0..nfine) | Lf[0..nfine)], each nm x nm
 // row-major.  STAGE: the operators are copied to LDS once per workgroup
 // (they fit for every type up to order 4); otherwise they are read through
-// the caches.  Lanes own mortar DOFs; workgroups stride over interfaces.
+// the caches.  A workgroup takes ipb = 256 / nm interfaces at a time, one
+// lane per (interface, mortar DOF), and strides over the batch.
 template <bool STAGE>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(256)
     mortar_kernel(int kind, FluxParams p, int nm, int nfine, const double *__restrict__ ops,
                   int64_t nif, const double *__restrict__ Uc, const double *__restrict__ Uf,
                   const double *__restrict__ nrm, double *__restrict__ rc,
@@ -260,57 +263,64 @@ __global__ void __launch_bounds__(128)
 {
     extern __shared__ double sm[];
     const int nmat = 3 + 2 * nfine, mm = nm * nm;
-    double *su = sm;                        // (1 + nfine) * nm inputs
-    double *sF = su + (size_t)(1 + nfine) * nm; // 2 * nm: flux, Minv * flux
+    const int ipb = 256 / nm;                  // interfaces per iteration
+    const int per = (1 + nfine) * nm + 2 * nm; // inputs, flux, M^-1 flux
+    double *sbuf = sm;                         // ipb * per
     const double *op = ops;
     if (STAGE) {
-        double *sops = sF + 2 * nm;
+        double *sops = sm + (size_t)ipb * per;
         for (int i = threadIdx.x; i < nmat * mm; i += blockDim.x)
             sops[i] = ops[i];
         op = sops;
     }
     const double *Minv = op, *Pc = op + mm, *Lc = op + 2 * mm, *Pf = op + 3 * mm,
                  *Lf = op + (size_t)(3 + nfine) * mm;
-    const int r = threadIdx.x;
-    for (int64_t it = blockIdx.x; it < nif; it += gridDim.x) {
+    const int g = threadIdx.x / nm, r = threadIdx.x - g * nm;
+    const bool lane_on = g < ipb;
+    double *su = sbuf + (size_t)(lane_on ? g : 0) * per; // Uc | Uf[nfine] | F | M^-1 F
+    double *sF = su + (1 + nfine) * nm;
+    for (int64_t i0 = (int64_t)blockIdx.x * ipb; i0 < nif; i0 += (int64_t)gridDim.x * ipb) {
+        const int64_t it = i0 + g;
+        const bool act = lane_on && it < nif;
+        __syncthreads(); // operators staged; the previous interfaces are done
+        if (act) {
+            su[r] = Uc[it * nm + r];
+            for (int f = 0; f < nfine; ++f)
+                su[nm + f * nm + r] = Uf[(it * nfine + f) * nm + r];
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < nm; i += blockDim.x)
-            su[i] = Uc[it * nm + i];
-        for (int i = threadIdx.x; i < nfine * nm; i += blockDim.x)
-            su[nm + i] = Uf[it * nfine * nm + i];
-        __syncthreads();
-        const double n[3] = {nrm[3 * it], nrm[3 * it + 1], nrm[3 * it + 2]};
-        if (r < nm) {
+        if (act) {
+            const double n[3] = {nrm[3 * it], nrm[3 * it + 1], nrm[3 * it + 2]};
             double uc = 0.0;
             for (int c = 0; c < nm; ++c)
                 uc += Pc[r * nm + c] * su[c];
             double uf = 0.0;
             for (int f = 0; f < nfine; ++f) {
-                double s = 0.0;
+                double t = 0.0;
                 for (int c = 0; c < nm; ++c)
-                    s += Pf[(size_t)f * mm + r * nm + c] * su[nm + f * nm + c];
-                uf += s;
+                    t += Pf[(size_t)f * mm + r * nm + c] * su[nm + f * nm + c];
+                uf += t;
             }
             sF[r] = scalar_flux(kind, uc, uf, n, p);
         }
         __syncthreads();
-        if (r < nm) {
+        if (act) {
             double mf = 0.0;
             for (int c = 0; c < nm; ++c)
                 mf += Minv[r * nm + c] * sF[c];
             sF[nm + r] = mf;
         }
         __syncthreads();
-        if (r < nm) {
-            double s = 0.0;
+        if (act) {
+            double t = 0.0;
             for (int c = 0; c < nm; ++c)
-                s += Lc[r * nm + c] * sF[nm + c];
-            rc[it * nm + r] = s;
+                t += Lc[r * nm + c] * sF[nm + c];
+            rc[it * nm + r] = t;
             for (int f = 0; f < nfine; ++f) {
-                double t = 0.0;
+                double v = 0.0;
                 for (int c = 0; c < nm; ++c)
-                    t += -Lf[(size_t)f * mm + r * nm + c] * sF[nm + c];
-                rf[(it * nfine + f) * nm + r] = t;
+                    v += -Lf[(size_t)f * mm + r * nm + c] * sF[nm + c];
+                rf[(it * nfine + f) * nm + r] = v;
             }
         }
     }
@@ -353,10 +363,11 @@ hipError_t launch_conforming(int kind, const FluxParams &p, int nf, const FaceRu
                              int64_t nfaces, const double *UL, const double *UR,
                              const double *nrm, double *rl, double *rr, hipStream_t s)
 {
-    if (nfaces <= 0)
+    const int64_t total = nfaces * nf;
+    if (total <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(conforming_kernel, dim3((unsigned)nfaces), dim3(128), 0, s, kind, p, nf,
-                       rule, nfaces, UL, UR, nrm, rl, rr);
+    hipLaunchKernelGGL(conforming_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       kind, p, nf, rule, nfaces, UL, UR, nrm, rl, rr);
     return hipGetLastError();
 }
 
@@ -366,14 +377,16 @@ hipError_t launch_mortar(int kind, const FluxParams &p, int nm, int nfine, const
 {
     if (nif <= 0)
         return hipSuccess;
-    const size_t base = sizeof(double) * ((size_t)(1 + nfine) * nm + 2 * (size_t)nm);
+    const int ipb = 256 / nm;
+    const size_t base = sizeof(double) * (size_t)ipb * ((size_t)(1 + nfine) * nm + 2 * (size_t)nm);
     const size_t opb = sizeof(double) * (size_t)(3 + 2 * nfine) * nm * nm;
-    const unsigned grid = (unsigned)(nif < 4096 ? nif : 4096);
+    const int64_t need = (nif + ipb - 1) / ipb;
+    const unsigned grid = (unsigned)(need < 4096 ? need : 4096);
     if (base + opb <= 64 * 1024)
-        hipLaunchKernelGGL(mortar_kernel<true>, dim3(grid), dim3(128), base + opb, s, kind, p, nm,
+        hipLaunchKernelGGL(mortar_kernel<true>, dim3(grid), dim3(256), base + opb, s, kind, p, nm,
                            nfine, ops, nif, Uc, Uf, nrm, rc, rf);
     else
-        hipLaunchKernelGGL(mortar_kernel<false>, dim3(grid), dim3(128), base, s, kind, p, nm,
+        hipLaunchKernelGGL(mortar_kernel<false>, dim3(grid), dim3(256), base, s, kind, p, nm,
                            nfine, ops, nif, Uc, Uf, nrm, rc, rf);
     return hipGetLastError();
 }

@@ -42,7 +42,7 @@ enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_G, S_NF };
 
 template <int N> struct Cfg {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-    static constexpr int lds_per_elem_fp64 = (S_NF * n3 + 2 * n2) * 8;
+    static constexpr int lds_per_elem_fp64 = (S_NF * n3 + 4 * n2) * 8; // slots + column integrals
     static constexpr int epb_thr = (128 / n2) > 0 ? 128 / n2 : 1;
     static constexpr int epb_lds = (65536 / lds_per_elem_fp64) > 0 ? 65536 / lds_per_elem_fp64 : 1;
     static constexpr int EPB = epb_thr < epb_lds ? epb_thr : epb_lds;
@@ -123,7 +123,7 @@ __device__ __forceinline__ void d2line(const TC *__restrict__ D, const TC *__res
     }
 }
 
-// The state lines a thread needs for phase 1.
+// The state lines a thread needs for phase P1.
 template <int n, class TC> struct Lines {
     TC hu[n], hv[n], et[n], ht[n], hs[n], hb[n];
 };

@@ -109,3 +109,35 @@ def test_error_before_initialize():
         s.compute_rhs(np.zeros(5 * 64))
     assert ei.value.status == _lib.DRF_ERR_STATE
     s.close()
+
+
+@pytest.mark.parametrize("prec", [_lib.PREC_FP64, _lib.PREC_MIXED])
+def test_launch_shape_does_not_change_results(prec):
+    """every element is computed the same way whatever the grid: all launch
+    shapes, and the autotuner's choice, give identical bits"""
+    import torch
+
+    case = synth.config1()
+    E, n = case.E, case.n
+    s = PrimitiveEquationsSolver(case.order, prec)
+    s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    dt = torch.float64 if prec == _lib.PREC_FP64 else torch.float32
+    U = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda().to(dt)
+    ref = torch.empty_like(U)
+    s.set_launch(0, True)
+    s.compute_rhs_dev(U, ref, _lib.LAYOUT_LINES)
+    out = torch.empty_like(U)
+    for wpc in (1, 2, 3, 8):
+        for remap in (False, True):
+            s.set_launch(wpc, remap)
+            out.zero_()
+            s.compute_rhs_dev(U, out, _lib.LAYOUT_LINES)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (wpc, remap)
+    wpc, remap, us = s.autotune(U, out, _lib.LAYOUT_LINES, reps=2)
+    assert 1 <= wpc <= 8 and us > 0
+    assert s.get_launch() == (wpc, remap)
+    s.compute_rhs_dev(U, out, _lib.LAYOUT_LINES)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    s.close()
