@@ -271,6 +271,37 @@ def measure_faces(stream):
     return out
 
 
+def measure_amr(stream):
+    """Solution transfer of one adapt cycle at the bench mesh size (32768
+    elements, N = 4, one field): transfer_solution with 10% of the elements
+    refined, and project_to_children (XYZ) of every element."""
+    import torch
+
+    from drifter_amd import amr
+
+    dev = "cuda"
+    order, E = 4, 32768
+    nd = (order + 1) ** 3
+    g = torch.Generator(device=dev).manual_seed(2)
+    U = torch.rand((E, nd), device=dev, dtype=torch.float64, generator=g)
+    parents = torch.arange(0, E, 10, device=dev, dtype=torch.int64)
+    R = parents.numel()
+    out = {}
+    ms = time_launches(lambda: amr.transfer_solution(order, U, parents, stream=stream), 20, 3,
+                       stream)
+    keep = E - R
+    moved = (2 * keep + R + 8 * R) * nd * 8 + R * 8
+    out["transfer_solution_32768_elems_10pct_n4_us"] = round(ms * 1e3, 1)
+    out["transfer_solution_gb_s"] = round(moved / (ms * 1e-3) / 1e9, 1)
+    sp = amr.SolutionProjection(order)
+    ms = time_launches(lambda: sp.project_to_children(U, amr.MASK_XYZ, stream=stream), 20, 3,
+                       stream)
+    out["project_to_children_32768_parents_n4_us"] = round(ms * 1e3, 1)
+    out["project_to_children_gb_s"] = round(9 * E * nd * 8 / (ms * 1e-3) / 1e9, 1)
+    log(f"amr transfer: {out}")
+    return out
+
+
 def measure_indicator(stream):
     """Learned indicator: training on 50 seeded coastline states (about 2M
     element/label pairs, 16 epochs), misclassification against the estimator on 5
@@ -391,6 +422,7 @@ def main():
         extras.update(measure_mode_split(case, stream))
         extras.update(measure_coastline(args, stream))
         extras.update(measure_faces(stream))
+        extras.update(measure_amr(stream))
         extras.update(measure_indicator(stream))
 
     cpu = None if args.no_cpu else cpu_baseline(case)

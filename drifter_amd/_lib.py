@@ -84,6 +84,11 @@ SIGNATURES = {
     "drf_estimator_destroy": (None, [_P]),
     "drf_element_char_length": (_I, [_L, _PD, _PD]),
     "drf_estimate_dev": (_I, [_P, _L, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "drf_project_to_children_dev": (_I, [_I, _I, _L, _P, _L, _P, _L, _P]),
+    "drf_project_from_children_dev": (_I, [_I, _I, _L, _P, _L, _P, _L, _P]),
+    "drf_transfer_solution_dev": (_I, [_I, _L, _P, _L, _L, _P, _P, _L, _P]),
+    "drf_interpolate_boxes_dev": (_I, [_I, _L, _L, _P, _P, _P, _L, _P, _L, _P]),
+    "drf_refine_transfer_map": (_I, [_L, _PD, _L, _PD, _P, _PD]),
     "drf_modal_matrices": (_I, [_I, _PD, _PD]),
     "drf_modal_coeffs_dev": (_I, [_I, _L, _P, _L, _P, _P]),
     "drf_modal_features_bf16_dev": (_I, [_I, _L, _P, _L, _D, _P, _P, _L, _I, _P]),

@@ -321,6 +321,57 @@ drf_status drf_estimate_dev(drf_estimator *est, int64_t n_elem, const void *fiel
                             void *mask_out, void *max_value, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Solution transfer on refinement (SolutionProjection,                */
+/* src/amr/refinement.cpp:217-371; transfer_solution :533-567)         */
+/* ------------------------------------------------------------------ */
+/* One nodal field per element (element e at U + e*elem_stride, device,
+ * FP64).  Values are the reference's interpolate_3d (:344-371) applied by
+ * sum factorisation: the same interpolant, summed in another order (within
+ * 1e-13 of max |U|). */
+
+/* project_to_children (:217-276) for n_parent parents at once: child c of
+ * parent p at children + (p*nchild + c)*child_stride, nchild = 2^popcount(mask),
+ * c = cx + nx (cy + ny cz) with cx ... the child's half along each refined
+ * axis.  mask: RefineMask bits 1 X, 2 Y, 4 Z (include/mesh/refine_mask.hpp). */
+drf_status drf_project_to_children_dev(int order, int mask, int64_t n_parent, const void *U,
+                                       int64_t elem_stride, void *children, int64_t child_stride,
+                                       void *stream);
+/* project_from_children (:278-342): parent node values from the child that
+ * holds each node (the upper half from 0 on), at its clamped child
+ * coordinate.  Children laid out as drf_project_to_children_dev writes them. */
+drf_status drf_project_from_children_dev(int order, int mask, int64_t n_parent,
+                                         const void *children, int64_t child_stride, void *U,
+                                         int64_t elem_stride, void *stream);
+/* transfer_solution (:533-567) into an empty output: the n_elem - n_refine
+ * elements that are not parents, in order, then for each parent in order its
+ * 8 children of an XYZ projection (the reference projects with
+ * RefineMask::XYZ whatever the refine mask, :560).  parents: device int64[n_refine],
+ * strictly ascending and < n_elem, as adapt() builds the refinement map
+ * (:432-466); U_new holds n_elem + 7 n_refine elements.  (adapt() passes its
+ * solution vector as the output without clearing it, :471, so there the
+ * result follows the old elements.) */
+drf_status drf_transfer_solution_dev(int order, int64_t n_elem, const void *U, int64_t elem_stride,
+                                     int64_t n_refine, const void *parents, void *U_new,
+                                     int64_t new_stride, void *stream);
+/* Transfer onto the refined mesh's own leaf order (not the reference's
+ * behaviour: it fixes the always-XYZ projection and the appended-children
+ * order of transfer_solution).  Output element k is the source element
+ * src[k] (device int64) evaluated on the sub-box boxes[6k..6k+5] (device
+ * double: lo, hi per axis, in the source's reference coordinates); a box of
+ * exactly [-1, 1]^3 is a copy, a source outside [0, n_elem) gives NaN. */
+drf_status drf_interpolate_boxes_dev(int order, int64_t n_out, int64_t n_elem, const void *src,
+                                     const void *boxes, const void *U, int64_t elem_stride,
+                                     void *U_new, int64_t new_stride, void *stream);
+/* Host: src and boxes of drf_interpolate_boxes_dev from the leaf bounds
+ * (E*6, as drf_mesh_elements writes them) before and after
+ * drf_mesh_refine (refine + balance, octree_adapter.cpp:184-203).  Leaves
+ * are numbered depth first, so each new leaf lies in one old leaf and the
+ * walk is linear; DRF_ERR_ARG when the new mesh is not a refinement of the
+ * old one. */
+drf_status drf_refine_transfer_map(int64_t n_old, const double *old_bounds, int64_t n_new,
+                                   const double *new_bounds, int64_t *src, double *boxes);
+
+/* ------------------------------------------------------------------ */
 /* Modal coefficients (no counterpart in the reference: the modal      */
 /* Vandermonde of include/dg/basis.hpp:40-42 is declared only)         */
 /* ------------------------------------------------------------------ */

@@ -172,6 +172,14 @@ double orc_bf16_round(double x);
 double orc_mlp_loss_grad(const double *params, int64_t B, const double *X, const double *y,
                          int quantize, double *pred, double *grad);
 
+/* ---- AMR solution transfer (SolutionProjection, transfer_solution) ---- */
+/* children: nchild x ndof, child index cx + nx (cy + ny cz); returns nchild */
+int orc_project_to_children(int order, int mask, const double *U, double *children);
+int orc_project_from_children(int order, int mask, const double *children, double *U);
+/* parents ascending; returns the number of elements written to U_new */
+int64_t orc_transfer_solution(int order, int64_t E, const double *U, int64_t n_ref,
+                              const int64_t *parents, double *U_new);
+
 #ifdef __cplusplus
 }
 #endif

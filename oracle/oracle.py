@@ -74,6 +74,10 @@ def load():
         "orc_modal_coeffs": (C.c_int, [C.c_int, C.c_int64, _PD, C.c_int64, _PD]),
         "orc_sigma_metrics": (None, [C.c_int64] + [_PD] * 13),
         "orc_bf16_round": (C.c_double, [C.c_double]),
+        "orc_project_to_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
+        "orc_project_from_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
+        "orc_transfer_solution": (C.c_int64, [C.c_int, C.c_int64, _PD, C.c_int64, C.c_void_p,
+                                              _PD]),
         "orc_mlp_loss_grad": (C.c_double, [_PD, C.c_int64, _PD, _PD, C.c_int, _PD, _PD]),
     }
     for name, (res, args) in sig.items():
@@ -347,3 +351,27 @@ def mlp_loss_grad(params, X, y, quantize=True, want_grad=True):
     loss = load().orc_mlp_loss_grad(_p(params), B, _p(X), _p(y), int(quantize), _p(pred),
                                     None if grad is None else _p(grad))
     return loss, pred, grad
+
+
+def project_to_children(order, mask, U):
+    nd = (order + 1) ** 3
+    out = np.zeros(8 * nd)
+    nc = load().orc_project_to_children(order, mask, _p(_v(U)), _p(out))
+    return out[: nc * nd].reshape(nc, nd)
+
+
+def project_from_children(order, mask, children):
+    nd = (order + 1) ** 3
+    out = np.zeros(nd)
+    load().orc_project_from_children(order, mask, _p(_v(children).reshape(-1)), _p(out))
+    return out
+
+
+def transfer_solution(order, U, parents):
+    """U (E, ndof); parents ascending -> (E - n + 8 n, ndof)"""
+    U = _v(U)
+    E, nd = U.shape
+    parents = np.ascontiguousarray(parents, dtype=np.int64)
+    out = np.zeros((E + 8 * len(parents), nd))
+    k = load().orc_transfer_solution(order, E, _p(U), len(parents), parents.ctypes.data, _p(out))
+    return out[:k]

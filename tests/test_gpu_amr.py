@@ -1,0 +1,219 @@
+"""GPU parity of the solution transfer against the oracle:
+SolutionProjection::project_to_children / project_from_children
+(src/amr/refinement.cpp:217-342) and AdaptiveMeshRefinement::transfer_solution
+(:533-567).  The kernels apply the reference's interpolant by sum
+factorisation, so values agree to rounding: within 1e-13 of the largest
+|U| per element (relative L2 well below 1e-12); copied elements are
+bit-exact.  The transfer onto the refined mesh's own leaf order is checked
+by exact reproduction of polynomials of degree <= N."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+
+    assert t.cuda.is_available()
+    return t
+
+
+def _close(got, want):
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want).max()
+    assert err <= TOL * scale, (err, scale)
+    rel = np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300)
+    assert rel <= 1e-12, rel
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("mask", [1, 3, 4, 6, 7])
+def test_project_to_children_matches_oracle(torch, orc, order, mask):
+    from drifter_amd import amr
+
+    nd = (order + 1) ** 3
+    P = 37
+    U = np.random.default_rng(order * 8 + mask).standard_normal((P, nd))
+    print(f"project_to_children order={order} mask={mask} P={P}", flush=True)
+    got = amr.SolutionProjection(order).project_to_children(torch.from_numpy(U).cuda(), mask)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    for p in range(P):
+        _close(got[p], orc.project_to_children(order, mask, U[p]))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 6, 8])
+@pytest.mark.parametrize("mask", [1, 2, 5, 7])
+def test_project_from_children_matches_oracle(torch, orc, order, mask):
+    from drifter_amd import amr
+
+    nd = (order + 1) ** 3
+    nc = amr.num_children(mask)
+    P = 23
+    C = np.random.default_rng(order * 8 + mask + 1).standard_normal((P, nc, nd))
+    print(f"project_from_children order={order} mask={mask} P={P}", flush=True)
+    got = amr.SolutionProjection(order).project_from_children(torch.from_numpy(C).cuda(), mask)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    for p in range(P):
+        _close(got[p], orc.project_from_children(order, mask, C[p]))
+
+
+@pytest.mark.parametrize("order", [1, 3, 4, 7])
+def test_transfer_solution_matches_oracle(torch, orc, order):
+    from drifter_amd import amr
+
+    nd = (order + 1) ** 3
+    rng = np.random.default_rng(order)
+    E = 300
+    U = rng.standard_normal((E, nd))
+    parents = np.sort(rng.choice(E, size=41, replace=False))
+    parents[0] = 0  # first and last elements refined too
+    parents[-1] = E - 1
+    print(f"transfer_solution order={order} E={E} refined={parents.size}", flush=True)
+    want = orc.transfer_solution(order, U, parents)
+    dU = torch.from_numpy(U).cuda()
+    got = amr.transfer_solution(order, dU, parents).cpu().numpy()
+    # the same with the parents already on the device
+    got2 = amr.transfer_solution(order, dU, torch.from_numpy(parents).cuda()).cpu().numpy()
+    assert got.shape == want.shape == (E + 7 * parents.size, nd)
+    keep = E - parents.size
+    np.testing.assert_array_equal(got[:keep], want[:keep])  # copies: bit-exact
+    _close(got[keep:], want[keep:])
+    np.testing.assert_array_equal(got, got2)
+
+
+def test_transfer_edge_cases(torch, orc):
+    from drifter_amd import amr
+
+    order, nd = 2, 27
+    U = np.random.default_rng(3).standard_normal((5, nd))
+    dU = torch.from_numpy(U).cuda()
+    print("transfer edge cases", flush=True)
+    np.testing.assert_array_equal(amr.transfer_solution(order, dU, []).cpu().numpy(), U)
+    allp = amr.transfer_solution(order, dU, np.arange(5)).cpu().numpy()
+    _close(allp, orc.transfer_solution(order, U, np.arange(5)))
+    one = amr.transfer_solution(order, dU[:1], [0]).cpu().numpy()
+    _close(one, orc.project_to_children(order, 7, U[0]))
+    empty = amr.transfer_solution(order, dU[:0], [])
+    assert tuple(empty.shape) == (0, nd)
+
+
+def test_strided_field_view(torch, orc):
+    """One field of the packed layout [e][field][ndof] (element stride 5 ndof)."""
+    from drifter_amd import amr
+
+    order, nd = 3, 64
+    full = np.random.default_rng(9).standard_normal((50, 5, nd))
+    dfull = torch.from_numpy(full).cuda()
+    view = dfull[:, 2, :]
+    assert view.stride(0) == 5 * nd
+    print("strided transfer", flush=True)
+    got = amr.transfer_solution(order, view, [4, 17, 30]).cpu().numpy()
+    want = orc.transfer_solution(order, full[:, 2, :].copy(), [4, 17, 30])
+    np.testing.assert_array_equal(got[:47], want[:47])
+    _close(got[47:], want[47:])
+    ch = amr.SolutionProjection(order).project_to_children(view, 5).cpu().numpy()
+    _close(ch[7], orc.project_to_children(order, 5, full[7, 2]))
+
+
+def _poly_field(order, bounds, coef):
+    """f(x, y, z) = sum_s prod_a polyval(coef[s, a], x_a) at every element's LGL nodes."""
+    import oracle
+
+    x, _ = oracle.lgl(order + 1)
+    out = []
+    for b in bounds:
+        m = [b[2 * a] + 0.5 * (x + 1.0) * (b[2 * a + 1] - b[2 * a]) for a in range(3)]
+        Z, Y, X = np.meshgrid(m[2], m[1], m[0], indexing="ij")
+        t = 0.0
+        for s in range(coef.shape[0]):
+            t = t + (np.polyval(coef[s, 0], X) * np.polyval(coef[s, 1], Y)
+                     * np.polyval(coef[s, 2], Z))
+        out.append(t.reshape(-1))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_transfer_to_refined_mesh_reproduces_polynomials(torch, order):
+    from drifter_amd import amr
+    from drifter_amd.mesh import OctreeAdapter
+
+    m = OctreeAdapter(0.0, 2.0, 0.0, 2.0, -1.0, 0.0)
+    m.build_uniform(4, 4, 2)
+    old_b, _, _ = m.elements()
+    coef = np.random.default_rng(order).standard_normal((2, 3, order + 1))
+    U = _poly_field(order, old_b, coef)
+    rng = np.random.default_rng(11)
+    els = np.sort(rng.choice(old_b.shape[0], size=7, replace=False))
+    masks = rng.choice([1, 2, 3, 4, 5, 6, 7], size=7)
+    m.refine(els, masks)  # refine + balance
+    new_b, _, _ = m.elements()
+    print(f"transfer to refined mesh order={order} E {old_b.shape[0]} -> {new_b.shape[0]}",
+          flush=True)
+    got = amr.transfer_to_refined_mesh(order, torch.from_numpy(U).cuda(), old_b, new_b)
+    got = got.cpu().numpy()
+    want = _poly_field(order, new_b, coef)
+    _close(got, want)
+    # unchanged leaves are exact copies
+    src, boxes = amr.refine_transfer_map(old_b, new_b)
+    full = np.all(boxes == np.array([-1, 1, -1, 1, -1, 1.0]), axis=1)
+    assert full.any()
+    np.testing.assert_array_equal(got[full], U[src[full]])
+
+
+def test_refined_mesh_transfer_is_a_reordering_of_the_reference_transfer(torch):
+    """One level of XYZ refinement: the fixed transfer holds exactly the
+    reference transfer's elements, with the children at their parents' places."""
+    from drifter_amd import amr
+    from drifter_amd.mesh import OctreeAdapter
+
+    order, nd = 3, 64
+    m = OctreeAdapter(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    m.build_uniform(4, 4, 4)
+    old_b, _, _ = m.elements()
+    E = old_b.shape[0]
+    U = np.random.default_rng(2).standard_normal((E, nd))
+    parents = np.array([5, 21, 22, 42])  # interior, no balance cascade at one level
+    m.refine(parents, [7] * parents.size)
+    new_b, _, _ = m.elements()
+    assert new_b.shape[0] == E + 7 * parents.size
+    dU = torch.from_numpy(U).cuda()
+    print("fixed vs reference transfer ordering", flush=True)
+    ref = amr.transfer_solution(order, dU, parents).cpu().numpy()
+    fix = amr.transfer_to_refined_mesh(order, dU, old_b, new_b).cpu().numpy()
+    src, _ = amr.refine_transfer_map(old_b, new_b)
+    keep = E - parents.size
+    # positions of the parents' children in the fixed order
+    for r, p in enumerate(parents):
+        k = np.nonzero(src == p)[0]
+        assert k.size == 8
+        np.testing.assert_array_equal(fix[k], ref[keep + 8 * r: keep + 8 * r + 8])
+    others = np.setdiff1d(np.arange(E), parents)
+    np.testing.assert_array_equal(fix[np.isin(src, others)], ref[:keep])
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_large_round_trip(torch, order):
+    """Full-size property: project_from_children(project_to_children(U)) = U
+    and the transfer of every element keeps each element's mean over its
+    children (sum of children's interpolants = parent's, checked by the
+    round trip) at 150k parents."""
+    from drifter_amd import amr
+
+    nd = (order + 1) ** 3
+    P = 150_000 if order == 3 else 60_000
+    g = torch.Generator(device="cuda").manual_seed(order)
+    U = torch.randn((P, nd), dtype=torch.float64, device="cuda", generator=g)
+    sp = amr.SolutionProjection(order)
+    print(f"round trip order={order} P={P}", flush=True)
+    ch = sp.project_to_children(U, 7)
+    back = sp.project_from_children(ch, 7)
+    err = (back - U).abs().max().item()
+    assert err <= TOL * U.abs().max().item() * 10, err
+    allp = amr.transfer_solution(order, U, torch.arange(P, device="cuda"))
+    assert torch.equal(allp.view(P, 8, nd), ch)
