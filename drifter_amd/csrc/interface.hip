@@ -10,10 +10,12 @@
 //   MortarSpace::compute_mortar_flux    src/dg/mortar.cpp:212-258
 //   LGL face extraction                 src/dg/basis_hexahedron.cpp:431-522
 // Pointwise fluxes and conforming faces run one face point per lane.  Mortars
-// run persistent workgroups over the interfaces of one connection type: the
-// type's projection and lift operators (built once on the host) are staged in
-// LDS when they fit, and every lane owns one mortar DOF of one of the
-// workgroup's interfaces.
+// of orders 1-3 run on the FP64 matrix cores: 16 interfaces per wavefront,
+// each operator product a 16 x 16 x (N+1)^2 GEMM with the operators in LDS.
+// Higher orders run persistent workgroups over the interfaces of one
+// connection type: the type's projection and lift operators (built once on
+// the host) are staged in LDS when they fit, and every lane owns one mortar
+// DOF of one of the workgroup's interfaces.
 #include "drf_internal.hpp"
 #include "interface.hpp"
 
@@ -326,6 +328,177 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// MortarSpace::compute_mortar_flux on the FP64 matrix cores, for nm =
+// (N+1)^2 <= 16 (orders 1-3).  A wavefront takes 16 interfaces; every
+// operator product is a 16 x 16 x nm GEMM, operator (rows = mortar DOFs,
+// zero-padded to 16 x KP in LDS) times the 16 interfaces' vectors, in
+// v_mfma_f64_16x16x4f64 steps of K = 4.  Lane l = col + 16 kq holds
+//   A: row m = col, K index 4c + kq of step c;   B: the same K index of
+//   interface col;   D: rows kq + 4 i (i < 4) of interface col
+// (the f64 variant interleaves its output rows; measured on gfx950).  Two
+// permutations make every lane's memory accesses contiguous: step c, lane
+// group kq contracts DOF KC kq + c (so a lane's KC input DOFs are
+// consecutive), and A row m is operator row 4 (m % 4) + m / 4 (so a lane's
+// four outputs are rows 4 kq .. 4 kq + 3).  The fine projections accumulate
+// into one chain (K = nfine nm); the flux is taken on the D layout, and M^-1 F
+// and the lifts read their B operand back from a per-wave 16 x 16 LDS tile.
+// Products round as the matrix cores round them (within 1e-13 of the
+// sequential sums).
+constexpr int kMortarWaves = 4; // wavefronts per workgroup, sharing one operator copy
+
+template <int NM, int NF>
+__global__ void __launch_bounds__(64 * kMortarWaves)
+    mortar_mfma_kernel(int kind, FluxParams p, const double *__restrict__ ops, int64_t nif,
+                       const double *__restrict__ Uc, const double *__restrict__ Uf,
+                       const double *__restrict__ nrm, double *__restrict__ rc,
+                       double *__restrict__ rf)
+{
+    constexpr int KC = (NM + 3) / 4, KP = 4 * KC, NMAT = 3 + 2 * NF, MM = NM * NM;
+    // one padded, permuted operator: 16 rows of KP, row stride KP + 1 (odd,
+    // so the 16 rows an A fragment reads fall in different LDS banks)
+    constexpr int RS = KP + 1, OPS = 16 * RS;
+    __shared__ double sOp[NMAT * OPS];
+    __shared__ double sT[kMortarWaves][16 * 16];
+    // sOp[mat][m][4c + kq] = Op[4 (m % 4) + m / 4][KC kq + c]
+    for (int q = threadIdx.x; q < NMAT * OPS; q += 64 * kMortarWaves) {
+        const int m = q / OPS, r = (q - m * OPS) / RS, j = q - m * OPS - r * RS;
+        const int row = 4 * (r & 3) + (r >> 2), k = KC * (j & 3) + (j >> 2);
+        sOp[q] = (j < KP && row < NM && k < NM) ? ops[m * MM + row * NM + k] : 0.0;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, col = l & 15, kq = l >> 4;
+    double *T = sT[w];
+    const double *Minv = sOp, *Pc = sOp + OPS, *Lc = sOp + 2 * OPS, *Pf = sOp + 3 * OPS,
+                 *Lf = sOp + (3 + NF) * OPS;
+    const int k0 = KC * kq; // this lane's first K index
+    for (int64_t i0 = ((int64_t)blockIdx.x * kMortarWaves + w) * 16; i0 < nif;
+         i0 += (int64_t)gridDim.x * kMortarWaves * 16) {
+        const int64_t it = i0 + col;
+        const bool valid = it < nif;
+        // the lane's KC consecutive DOFs of every input face
+        double bc[KC], bf[NF][KC];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            bc[c] = 0.0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                bf[f][c] = 0.0;
+        }
+        if (valid) {
+            const double *uc = Uc + it * NM + k0, *uf = Uf + it * NF * NM + k0;
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+                if (NM % 4 == 0 || k0 + c < NM) {
+                    bc[c] = uc[c];
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        bf[f][c] = uf[f * NM + c];
+                }
+        }
+        dbl4 ac = {0.0, 0.0, 0.0, 0.0}, af = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            ac = __builtin_amdgcn_mfma_f64_16x16x4f64(Pc[col * RS + 4 * c + kq], bc[c], ac, 0, 0,
+                                                      0);
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+                af = __builtin_amdgcn_mfma_f64_16x16x4f64(Pf[f * OPS + col * RS + 4 * c + kq],
+                                                          bf[f][c], af, 0, 0, 0);
+        double n3[3] = {0.0, 0.0, 0.0};
+        if (valid) {
+            n3[0] = nrm[3 * it];
+            n3[1] = nrm[3 * it + 1];
+            n3[2] = nrm[3 * it + 2];
+        }
+        // D register i of this lane is mortar DOF 4 kq + i
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * kq + i;
+            T[row * 16 + col] =
+                (valid && row < NM) ? scalar_flux(kind, ac[i], af[i], n3, p) : 0.0;
+        }
+        wave_sync_lds();
+        dbl4 am = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            am = __builtin_amdgcn_mfma_f64_16x16x4f64(Minv[col * RS + 4 * c + kq],
+                                                      T[(k0 + c) * 16 + col], am, 0, 0, 0);
+        wave_sync_lds(); // every lane has read F before M^-1 F overwrites it
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            T[(4 * kq + i) * 16 + col] = am[i];
+        wave_sync_lds();
+        double bm[KC];
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            bm[c] = T[(k0 + c) * 16 + col];
+        dbl4 ar = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            ar = __builtin_amdgcn_mfma_f64_16x16x4f64(Lc[col * RS + 4 * c + kq], bm[c], ar, 0, 0,
+                                                      0);
+        if (valid)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * kq + i < NM)
+                    rc[it * NM + 4 * kq + i] = ar[i];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            dbl4 a2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+                a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lf[f * OPS + col * RS + 4 * c + kq],
+                                                          bm[c], a2, 0, 0, 0);
+            if (valid)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (4 * kq + i < NM)
+                        rf[(it * NF + f) * NM + 4 * kq + i] = -a2[i];
+        }
+        wave_sync_lds(); // the tile is rewritten by the next interfaces
+    }
+}
+
+template <int NM>
+hipError_t launch_mortar_mfma(int kind, const FluxParams &p, int nfine, const double *ops,
+                              int64_t nif, const double *Uc, const double *Uf, const double *nrm,
+                              double *rc, double *rf, hipStream_t s)
+{
+    const int64_t need = (nif + 16 * kMortarWaves - 1) / (16 * kMortarWaves);
+    const dim3 g((unsigned)(need < 4096 ? need : 4096)), b(64 * kMortarWaves);
+    switch (nfine) {
+    case 1:
+        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 1>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
+                           rc, rf);
+        break;
+    case 2:
+        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 2>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
+                           rc, rf);
+        break;
+    case 3:
+        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 3>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
+                           rc, rf);
+        break;
+    case 4:
+        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 4>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
+                           rc, rf);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // Face traces of a nodal field: out[k][q] = field[elems[k] * es + node(face[k], q)].
 __global__ void __launch_bounds__(256)
     gather_faces_kernel(int n, int64_t nelem, int64_t es, int64_t nfaces,
@@ -377,6 +550,12 @@ hipError_t launch_mortar(int kind, const FluxParams &p, int nm, int nfine, const
 {
     if (nif <= 0)
         return hipSuccess;
+    switch (nm) { // orders 1-3 on the matrix cores
+    case 4: return launch_mortar_mfma<4>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 9: return launch_mortar_mfma<9>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 16: return launch_mortar_mfma<16>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    default: break;
+    }
     const int ipb = 256 / nm;
     const size_t base = sizeof(double) * (size_t)ipb * ((size_t)(1 + nfine) * nm + 2 * (size_t)nm);
     const size_t opb = sizeof(double) * (size_t)(3 + 2 * nfine) * nm * nm;
