@@ -43,15 +43,21 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__device__ __forceinline__ float gelu(float z)
+// GELU, tanh form (Hendrycks & Gimpel): 0.5 z (1 + tanh(u)) with
+// u = sqrt(2/pi) (z + 0.044715 z^3), evaluated as z sigma(2u) since
+// 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)): one exp2 and one reciprocal.
+// The derivative is sigma + 2 z sigma (1 - sigma) u'(z).
+constexpr float kLog2e = 1.4426950408889634f;
+__device__ __forceinline__ float gelu_sigma(float z)
 {
-    const float t = tanhf(kC * (z + kA * z * z * z));
-    return 0.5f * z * (1.0f + t);
+    const float e = __builtin_amdgcn_exp2f(-2.0f * kLog2e * kC * (z + kA * z * z * z));
+    return __builtin_amdgcn_rcpf(1.0f + e); // e = inf gives 0
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 __device__ __forceinline__ float gelu_d(float z)
 {
-    const float t = tanhf(kC * (z + kA * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kC * (1.0f + 3.0f * kA * z * z);
+    const float sg = gelu_sigma(z);
+    return sg + 2.0f * z * sg * (1.0f - sg) * kC * (1.0f + 3.0f * kA * z * z);
 }
 
 // acc += A[m0:m0+16][0:K] * B[0:K][n0:n0+16]; A(m,k) = a[m*am + k*ak],
@@ -190,55 +196,134 @@ __device__ void forward_tile(Smem<R, TRAIN> &s, float b3)
     __syncthreads();
 }
 
-// Persistent inference: weights staged once per workgroup; the next tile's
-// rows are loaded into registers while the current tile is computed.
-template <int R>
-__global__ void __launch_bounds__(256)
-    mlp_forward_kernel(int64_t n, const uint16_t *__restrict__ X, const float *__restrict__ w,
-                       const uint16_t *__restrict__ wq, float *__restrict__ pred)
+// Inference, one wavefront per 16-row tile with no workgroup barriers.  The
+// layers are computed transposed, H^T = W X^T, so a layer's accumulator
+// (lane l: hidden units 16 mb + 4 (l / 16) + t of row l % 16) is exactly the
+// next layer's B operand (K index 4 (l / 16) + t of K-step mb, column l % 16):
+// activations never leave registers.  W2 lives in registers (its 16 A
+// fragments); W1's 32 fragments are read from LDS, laid out per (mb, step,
+// lane) so every read is one conflict-free 8-byte access.  Layer 1's K order
+// is permuted (step s of lane group g contracts inputs 32 g + 4 s .. + 3) so
+// each lane's 32 inputs of a row are contiguous: four 16-byte loads, issued
+// for the next tile before the current one is computed.
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b)
 {
-    __shared__ Smem<R, false> s;
-    stage_weights(s, w, wq);
-    const float b3 = w[OB3];
-    const int64_t ntiles = (n + R - 1) / R;
-    constexpr int VPR = NI / 8;                 // 16-byte vectors per row
-    constexpr int VPT = (R * VPR + 255) / 256;  // vectors per thread per tile
-    uint4 nxt[VPT];
-    auto fetch = [&](int64_t r0) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2v)); // RNE
+}
+
+constexpr int kInferWaves = 4;
+
+__global__ void __launch_bounds__(64 * kInferWaves)
+    mlp_infer_kernel(int64_t n, const uint16_t *__restrict__ X, const float *__restrict__ w,
+                     const uint16_t *__restrict__ wq, float *__restrict__ pred)
+{
+    // W1 fragments: f1[(mb * 8 + s) * 64 + lane][t] = W1[16 mb + lane % 16][32 (lane / 16) + 4 s + t]
+    __shared__ bf16x4 f1[4 * 8 * 64];
+    __shared__ __align__(16) float b1s[NH], b2s[NH], w3s[NH];
+    for (int q = threadIdx.x; q < 4 * 8 * 64; q += 64 * kInferWaves) {
+        const int ln = q & 63, ms = q >> 6, mb = ms >> 3, st = ms & 7;
+        const uint16_t *src = wq + (16 * mb + (ln & 15)) * NI + 32 * (ln >> 4) + 4 * st;
+        bf16x4 v;
 #pragma unroll
-        for (int q = 0; q < VPT; ++q) {
-            const int v = threadIdx.x + q * 256;
-            const int r = v / VPR, c = (v % VPR) * 8;
-            nxt[q] = make_uint4(0, 0, 0, 0);
-            if (v < R * VPR && r0 + r < n)
-                nxt[q] = *reinterpret_cast<const uint4 *>(X + (r0 + r) * NI + c);
+        for (int t = 0; t < 4; ++t)
+            v[t] = (short)src[t];
+        f1[q] = v;
+    }
+    for (int q = threadIdx.x; q < NH; q += 64 * kInferWaves) {
+        b1s[q] = w[OB1 + q];
+        b2s[q] = w[OB2 + q];
+        w3s[q] = bf2f(wq[NH * NI + NH * NH + q]);
+    }
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    // W2 fragments: A[m = 16 mb + r][k = 16 s + 4 g + t]
+    bf16x4 f2[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+            f2[mb][st] = *reinterpret_cast<const bf16x4 *>(wq + NH * NI + (16 * mb + r) * NH +
+                                                          16 * st + 4 * g);
+    const float b3 = w[OB3];
+    __syncthreads();
+    const int64_t ntiles = (n + 15) / 16;
+    const int64_t nwaves = (int64_t)gridDim.x * kInferWaves;
+    int64_t tile = (int64_t)blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+    uint4 xv[4]; // this lane's 32 inputs of row 16 tile + r
+    auto fetch = [&](int64_t tl) {
+        const int64_t row = 16 * tl + r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xv[q] = make_uint4(0, 0, 0, 0);
+        if (row < n) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(X + row * NI + 32 * g);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                xv[q] = src[q];
         }
     };
-    int64_t tile = blockIdx.x;
     if (tile < ntiles)
-        fetch(tile * R);
-    for (; tile < ntiles; tile += gridDim.x) {
-        const int64_t r0 = tile * R;
-        __syncthreads(); // previous tile's rows and pred consumed, weights staged
+        fetch(tile);
+    for (; tile < ntiles; tile += nwaves) {
+        uint4 cur[4];
 #pragma unroll
-        for (int q = 0; q < VPT; ++q) {
-            const int v = threadIdx.x + q * 256;
-            if (v < R * VPR) {
-                const int r = v / VPR, c = (v % VPR) * 8;
-                uint32_t *dst = reinterpret_cast<uint32_t *>(&s.x[r][c]);
-                dst[0] = nxt[q].x;
-                dst[1] = nxt[q].y;
-                dst[2] = nxt[q].z;
-                dst[3] = nxt[q].w;
-            }
+        for (int q = 0; q < 4; ++q)
+            cur[q] = xv[q];
+        if (tile + nwaves < ntiles)
+            fetch(tile + nwaves);
+        // layer 1: z1^T = W1 X^T, 4 blocks of 16 hidden units x 16 rows
+        f32x4 acc[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+            acc[mb] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const uint4 &c = cur[st >> 1];
+            bf16x4 bx;
+            const uint32_t lo = (st & 1) ? c.z : c.x, hi = (st & 1) ? c.w : c.y;
+            bx = __builtin_bit_cast(bf16x4, ((uint64_t)hi << 32) | lo);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb)
+                acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(f1[(mb * 8 + st) * 64 + lane],
+                                                                    bx, acc[mb], 0, 0, 0);
         }
-        if (tile + gridDim.x < ntiles)
-            fetch((tile + gridDim.x) * R);
-        __syncthreads();
-        forward_tile(s, b3);
-        for (int r = threadIdx.x; r < R; r += blockDim.x)
-            if (r0 + r < n)
-                pred[r0 + r] = s.pred[r];
+        // bias + GELU, rounded to BF16: layer 2's B fragments
+        bf16x4 a1[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const float4 bb = *reinterpret_cast<const float4 *>(&b1s[16 * mb + 4 * g]);
+            const uint32_t lo = pack_bf16(gelu(acc[mb][0] + bb.x), gelu(acc[mb][1] + bb.y));
+            const uint32_t hi = pack_bf16(gelu(acc[mb][2] + bb.z), gelu(acc[mb][3] + bb.w));
+            a1[mb] = __builtin_bit_cast(bf16x4, ((uint64_t)hi << 32) | lo);
+        }
+        // layer 2
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int st = 0; st < 4; ++st)
+                a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(f2[mb][st], a1[st], a, 0, 0, 0);
+            acc[mb] = a;
+        }
+        // output layer on the BF16-rounded activations, then the 4 lane groups
+        float p = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const float4 bb = *reinterpret_cast<const float4 *>(&b2s[16 * mb + 4 * g]);
+            const float4 ww = *reinterpret_cast<const float4 *>(&w3s[16 * mb + 4 * g]);
+            const uint32_t lo = pack_bf16(gelu(acc[mb][0] + bb.x), gelu(acc[mb][1] + bb.y));
+            const uint32_t hi = pack_bf16(gelu(acc[mb][2] + bb.z), gelu(acc[mb][3] + bb.w));
+            p += ww.x * __uint_as_float(lo << 16) + ww.y * __uint_as_float(lo & 0xffff0000u) +
+                 ww.z * __uint_as_float(hi << 16) + ww.w * __uint_as_float(hi & 0xffff0000u);
+        }
+        p += __shfl_xor(p, 16);
+        p += __shfl_xor(p, 32);
+        const int64_t row = 16 * tile + r;
+        if (g == 0 && row < n)
+            pred[row] = p + b3;
     }
 }
 
@@ -393,8 +478,11 @@ __global__ void __launch_bounds__(256)
                       float *__restrict__ m, float *__restrict__ v, uint16_t *__restrict__ wq)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    // Sums in partial order (deterministic); unrolled so that the loads of
+    // a group are in flight together.
     if (p == 0 && loss) {
         float a = 0.f;
+#pragma unroll 16
         for (int i = 0; i < nparts; ++i)
             a += lpart[i];
         *loss = a * inv_b;
@@ -402,6 +490,7 @@ __global__ void __launch_bounds__(256)
     if (p >= NP)
         return;
     float g = 0.f;
+#pragma unroll 16
     for (int i = 0; i < nparts; ++i)
         g += partial[(size_t)i * NP + p];
     if (grad)
@@ -433,7 +522,7 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-constexpr int kRowsFwd = 32, kRowsTrain = 32;
+constexpr int kRowsTrain = 32;
 
 } // namespace
 
@@ -448,23 +537,23 @@ hipError_t launch_mlp_forward(int64_t n, const uint16_t *X, const float *w, cons
 {
     if (n <= 0)
         return hipSuccess;
-    const int64_t tiles = (n + kRowsFwd - 1) / kRowsFwd;
-    // persistent: a few resident workgroups per CU (LDS-bound), weights
-    // staged once per workgroup
+    const int64_t tiles = (n + 15) / 16;
+    // persistent wavefronts: as many workgroups as are resident at once
     static const int cap = [] {
         int dev = 0, per = 1;
         hipDeviceProp_t p;
         int cus = 256;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
             cus = p.multiProcessorCount;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mlp_forward_kernel<kRowsFwd>, 256,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mlp_infer_kernel, 64 * kInferWaves,
                                                          0) != hipSuccess ||
             per < 1)
             per = 1;
         return cus * per;
     }();
-    const unsigned grid = (unsigned)(tiles < cap ? tiles : cap);
-    hipLaunchKernelGGL(mlp_forward_kernel<kRowsFwd>, dim3(grid), dim3(256), 0, s, n, X, w, wq,
+    const int64_t need = (tiles + kInferWaves - 1) / kInferWaves;
+    const unsigned grid = (unsigned)(need < cap ? need : cap);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(64 * kInferWaves), 0, s, n, X, w, wq,
                        pred);
     return hipGetLastError();
 }
