@@ -144,6 +144,35 @@ def measure_extras(args, case, s64, U, r64, stream, tune=None):
     log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
         f"{ms_rk_m:.3f} ms")
 
+    # the driver's step with the CFL dt (SimulationDriver::step): host loop
+    # (wave speed to the host, compute_dt, fused step) vs device-resident
+    # (dt on the device, eager and as one graph of 20 steps); dt_max 1e-5
+    # keeps the stiff N = 4 state stable
+    import time
+
+    lib = _lib.load()
+    nstep, dx = 20, 2.0 / 5
+    Uw = U.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nstep):
+        c = s64.max_wave_speed_dev(Uw, _lib.LAYOUT_LINES, stream)
+        dtk = lib.drf_compute_dt(c, dx, 0.5, 1e-8, 1e-5)
+        s64.rk_step_dev(_lib.RK_RK4, 0.0, dtk, Uw, _lib.LAYOUT_LINES, True, stream)
+    torch.cuda.synchronize()
+    ms_host = (time.perf_counter() - t0) * 1e3 / nstep
+    Uw = U.clone()
+    dtd = torch.zeros(1, dtype=torch.float64, device=U.device)
+    td = torch.zeros(1, dtype=torch.float64, device=U.device)
+    ms_dev = time_launches(lambda: s64.rk_step_adaptive_dev(
+        _lib.RK_RK4, Uw, dtd, td, cfl=0.5, dt_min=1e-8, dt_max=1e-5, layout=_lib.LAYOUT_LINES,
+        stream=stream), nstep, 2, stream)
+    g = s64.capture_adaptive_steps(_lib.RK_RK4, Uw, dtd, td, n_steps=nstep, cfl=0.5, dt_min=1e-8,
+                                   dt_max=1e-5, layout=_lib.LAYOUT_LINES)
+    ms_graph = time_launches(lambda: g.launch(stream), 3, 1, stream) / nstep
+    log(f"adaptive RK4 step: host loop {ms_host:.3f} ms, device {ms_dev:.3f} ms, "
+        f"graph {ms_graph:.3f} ms")
+
     # mixed precision vs FP64 after 1 and 100 fused RK4 steps from the same
     # state (dt = 1e-5: the reference-coordinate horizontal diffusion
     # H nu_h D^2 is stiff at N = 4; the FP64 GPU path is the reference here,
@@ -173,6 +202,9 @@ def measure_extras(args, case, s64, U, r64, stream, tune=None):
         "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
         "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
         "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
+        "adaptive_rk4_step_ms_host_loop": round(ms_host, 4),
+        "adaptive_rk4_step_ms_device": round(ms_dev, 4),
+        "adaptive_rk4_step_ms_graph": round(ms_graph, 4),
     }
 
 

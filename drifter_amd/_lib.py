@@ -58,6 +58,8 @@ SIGNATURES = {
     "drf_mode_split_step": (_I, [_P, _D, _I, _PD]),
     "drf_mode_split_step_dev": (_I, [_P, _D, _I, _P, _P]),
     "drf_graph_rk": (_I, [_P, _I, _D, _P, _I, _I, _I, C.POINTER(_P)]),
+    "drf_rk_step_adaptive_dev": (_I, [_P, _I, _D, _D, _D, _D, _P, _I, _P, _P, _P]),
+    "drf_graph_rk_adaptive": (_I, [_P, _I, _D, _D, _D, _D, _P, _I, _I, _P, _P, C.POINTER(_P)]),
     "drf_graph_mode_split": (_I, [_P, _D, _I, _P, _I, C.POINTER(_P)]),
     "drf_graph_launch": (_I, [_P, _P]),
     "drf_graph_destroy": (None, [_P]),

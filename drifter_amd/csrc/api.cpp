@@ -132,7 +132,8 @@ bool lane_offsets_fit(const drf_ctx *c, int layout)
 // One volume-kernel launch at the context's precision.
 drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode, void *R,
                   const void *U0, void *Y, void *A, const void *X, double c0, double c1,
-                  double c2, double c3, hipStream_t s)
+                  double c2, double c3, hipStream_t s, const double *dtp = nullptr,
+                  double dtq = 1.0)
 {
     DRF_REQUIRE(lane_offsets_fit(c, layout), DRF_ERR_UNSUPPORTED,
                 "mesh too large for 32-bit per-field offsets in this layout (use LINES or SOA)");
@@ -150,6 +151,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c1 = c1;
         epi.c2 = c2;
         epi.c3 = c3;
+        epi.dtp = dtp;
+        epi.dtq = dtq;
         VolGeom<double> geo{c->E, coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
                             c->f0, c->wg_per_cu, c->xcd_remap};
         err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
@@ -167,6 +170,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.c1 = c1;
         epi.c2 = c2;
         epi.c3 = c3;
+        epi.dtp = dtp;
+        epi.dtq = dtq;
         VolGeom<float> geo{c->E, coef_view<float>(c->d_h32, c), coef_view<float>(c->d_f32, c),
                            c->f0, c->wg_per_cu, c->xcd_remap};
         err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
@@ -302,10 +307,17 @@ drf_status rk_unfused(drf_ctx *c, int scheme, double dt, double *U, int layout, 
 }
 
 // Fused stepping: each stage's vector update runs in the RHS kernel epilogue.
-drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipStream_t s)
+// Every dt-proportional coefficient is (m dt) / q; with dtp (device dt) the
+// kernel forms it from *dtp with the same operations, so both give the same
+// bits for the same dt.
+drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipStream_t s,
+                    const double *dtp = nullptr)
 {
-    if (scheme == DRF_RK_DP45)
+    if (scheme == DRF_RK_DP45) {
+        if (dtp)
+            return fail(DRF_ERR_UNSUPPORTED, "device time steps need a fused scheme (not DP45)");
         return rk_unfused(c, scheme, dt, (double *)U, layout, s);
+    }
     const int64_t len = state_len(c);
     const size_t es = elem_size(c);
     drf_status st = ensure_work(c, es * len * 2);
@@ -313,28 +325,31 @@ drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipS
         return st;
     char *w = (char *)c->d_work;
     void *Ut = w, *A = w + es * len;
+    // coefficient m dt / q: its value (host dt) or its multiplier (device dt)
+    auto D = [&](double m, double q) { return dtp ? m : (m * dt) / q; };
+    auto V = [&](const void *Uin, int mode, int acc, const void *U0, void *Y, void *Acc,
+                 const void *X, double c0, double c1, double c2, double c3, double q) {
+        return volume(c, Uin, layout, mode, acc, nullptr, U0, Y, Acc, X, c0, c1, c2, c3, s, dtp,
+                      q);
+    };
     switch (scheme) {
     case DRF_RK_EULER:
-        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, U, nullptr, nullptr, 0, dt, 0, 0, s));
+        TRY(V(U, EPI_STAGE, 0, U, U, nullptr, nullptr, 0, D(1.0, 1.0), 0, 0, 1.0));
         break;
     case DRF_RK_RK2:
-        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, Ut, nullptr, nullptr, 0, 0.5 * dt, 0,
-                   0, s));
-        TRY(volume(c, Ut, layout, EPI_STAGE, 0, nullptr, U, U, nullptr, nullptr, 0, dt, 0, 0, s));
+        TRY(V(U, EPI_STAGE, 0, U, Ut, nullptr, nullptr, 0, D(0.5, 1.0), 0, 0, 1.0));
+        TRY(V(Ut, EPI_STAGE, 0, U, U, nullptr, nullptr, 0, D(1.0, 1.0), 0, 0, 1.0));
         break;
     case DRF_RK_SSPRK3:
-        TRY(volume(c, U, layout, EPI_STAGE, 0, nullptr, U, Ut, nullptr, nullptr, 0, dt, 0, 0, s));
-        TRY(volume(c, Ut, layout, EPI_COMB, 0, nullptr, U, Ut, nullptr, Ut, 0.75, 0.25, dt, 0, s));
-        TRY(volume(c, Ut, layout, EPI_COMB, 0, nullptr, U, U, nullptr, Ut, 1.0 / 3.0, 2.0 / 3.0,
-                   dt, 0, s));
+        TRY(V(U, EPI_STAGE, 0, U, Ut, nullptr, nullptr, 0, D(1.0, 1.0), 0, 0, 1.0));
+        TRY(V(Ut, EPI_COMB, 0, U, Ut, nullptr, Ut, 0.75, 0.25, D(1.0, 1.0), 0, 1.0));
+        TRY(V(Ut, EPI_COMB, 0, U, U, nullptr, Ut, 1.0 / 3.0, 2.0 / 3.0, D(1.0, 1.0), 0, 1.0));
         break;
     case DRF_RK_RK4:
-        TRY(volume(c, U, layout, EPI_STAGE, 1, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 0, 0, s));
-        TRY(volume(c, Ut, layout, EPI_STAGE_ACC, 0, nullptr, U, Ut, A, nullptr, 0, 0.5 * dt, 2.0,
-                   0, s));
-        TRY(volume(c, Ut, layout, EPI_STAGE_ACC, 0, nullptr, U, Ut, A, nullptr, 0, dt, 2.0, 0, s));
-        TRY(volume(c, Ut, layout, EPI_RK4FIN, 0, nullptr, U, U, A, nullptr, 0, dt / 6.0, 0, 0,
-                   s));
+        TRY(V(U, EPI_STAGE, 1, U, Ut, A, nullptr, 0, D(0.5, 1.0), 0, 0, 1.0));
+        TRY(V(Ut, EPI_STAGE_ACC, 0, U, Ut, A, nullptr, 0, D(0.5, 1.0), 2.0, 0, 1.0));
+        TRY(V(Ut, EPI_STAGE_ACC, 0, U, Ut, A, nullptr, 0, D(1.0, 1.0), 2.0, 0, 1.0));
+        TRY(V(Ut, EPI_RK4FIN, 0, U, U, A, nullptr, 0, D(1.0, 6.0), 0, 0, 6.0));
         break;
     case DRF_RK_LSRK45: {
         static const double Ac[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471,
@@ -342,8 +357,8 @@ drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipS
         static const double Bc[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
                                      0.6994504559488, 0.1530572479681};
         for (int k = 0; k < 5; ++k)
-            TRY(volume(c, U, layout, EPI_LSRK, k == 0 ? 1 : 2, nullptr, nullptr, U, A, U, 0, dt,
-                       Ac[k], Bc[k], s));
+            TRY(V(U, EPI_LSRK, k == 0 ? 1 : 2, nullptr, U, A, U, 0, D(1.0, 1.0), Ac[k], Bc[k],
+                  1.0));
         break;
     }
     default:
@@ -370,6 +385,42 @@ drf_status wave_speed(drf_ctx *c, const void *U, int layout, double *out, hipStr
         e = hipStreamSynchronize(s);
     if (e != hipSuccess)
         return fail(DRF_ERR_HIP, std::string("wave speed: ") + hipGetErrorString(e));
+    return DRF_OK;
+}
+
+// SimulationDriver::step without leaving the device: the wave-speed maximum
+// of U into d_red[0], dt and time on the device, then the fused RK step that
+// reads dt from *dt_dev.
+drf_status adaptive_step(drf_ctx *c, int scheme, double cfl, double dx_min, double dt_min,
+                         double dt_max, void *U, int layout, double *dt_dev, double *t_dev,
+                         hipStream_t s)
+{
+    const int64_t npart = std::min<int64_t>(2048, (c->E + 3) / 4);
+    drf_status st = ensure_red(c, npart + 1);
+    if (st)
+        return st;
+    hipError_t e = launch_wave_speed(c->order, c->E, cview<double>(U, layout, c),
+                                     coef_view<double>(c->d_h, c), c->d_red + 1, npart, s);
+    if (e == hipSuccess)
+        e = launch_max_reduce(c->d_red + 1, npart, c->d_red, s);
+    if (e == hipSuccess)
+        e = launch_adaptive_dt(c->d_red, dx_min, cfl, dt_min, dt_max, dt_dev, t_dev, s);
+    if (e != hipSuccess)
+        return fail(DRF_ERR_HIP, std::string("adaptive dt: ") + hipGetErrorString(e));
+    return rk_fused(c, scheme, 0.0, U, layout, s, dt_dev);
+}
+
+drf_status check_adaptive(drf_ctx *c, int scheme, void *U, int layout, void *dt_dev)
+{
+    drf_status st = check_ready(c);
+    if (st)
+        return st;
+    DRF_REQUIRE(U && dt_dev, DRF_ERR_ARG, "null state or dt pointer");
+    DRF_REQUIRE(valid_layout(layout), DRF_ERR_ARG, "unknown layout");
+    DRF_REQUIRE(scheme >= DRF_RK_EULER && scheme < DRF_RK_DP45, DRF_ERR_UNSUPPORTED,
+                "device time steps need a fused scheme (Euler, RK2, SSPRK3, RK4, LSRK45)");
+    DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
+                "device time steps on FP64 contexts (the wave speed is FP64)");
     return DRF_OK;
 }
 
@@ -731,6 +782,17 @@ drf_status drf_rk_step_dev(drf_ctx *c, int scheme, double t, double dt, void *U,
     return rk_unfused(c, scheme, dt, (double *)U, layout, s);
 }
 
+drf_status drf_rk_step_adaptive_dev(drf_ctx *c, int scheme, double cfl, double dx_min,
+                                    double dt_min, double dt_max, void *U, int layout,
+                                    void *dt_dev, void *time_dev, void *stream)
+{
+    drf_status st = check_adaptive(c, scheme, U, layout, dt_dev);
+    if (st)
+        return st;
+    return adaptive_step(c, scheme, cfl, dx_min, dt_min, dt_max, U, layout, (double *)dt_dev,
+                         (double *)time_dev, pick_stream(c, stream));
+}
+
 drf_status drf_rk_step(drf_ctx *c, int scheme, double t, double dt, double *U)
 {
     drf_status st = check_ready(c);
@@ -869,6 +931,26 @@ int64_t drf_num_elements(const drf_ctx *c) { return c ? c->E : 0; }
 int drf_order(const drf_ctx *c) { return c ? c->order : 0; }
 
 // ---------------- HIP graphs of repeated steps ----------------
+
+drf_status drf_graph_rk_adaptive(drf_ctx *c, int scheme, double cfl, double dx_min,
+                                 double dt_min, double dt_max, void *U, int layout, int n_steps,
+                                 void *dt_dev, void *time_dev, drf_graph **out)
+{
+    DRF_REQUIRE(out, DRF_ERR_ARG, "null output");
+    *out = nullptr;
+    drf_status st = check_adaptive(c, scheme, U, layout, dt_dev);
+    if (st)
+        return st;
+    DRF_REQUIRE(n_steps >= 1 && n_steps <= 100000, DRF_ERR_ARG, "n_steps must be 1..100000");
+    // the reduction buffer is the context's: size it before capturing
+    st = ensure_red(c, std::min<int64_t>(2048, (c->E + 3) / 4) + 1);
+    if (st)
+        return st;
+    return capture_graph(c, elem_size(c) * state_len(c) * 2, n_steps, out, [&](hipStream_t cs) {
+        return adaptive_step(c, scheme, cfl, dx_min, dt_min, dt_max, U, layout,
+                             (double *)dt_dev, (double *)time_dev, cs);
+    });
+}
 
 drf_status drf_graph_rk(drf_ctx *c, int scheme, double dt, void *U, int layout, int fused,
                         int n_steps, drf_graph **out)

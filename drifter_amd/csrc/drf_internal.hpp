@@ -97,6 +97,11 @@ template <class T> struct Epi {
     View<T> A;        // accumulator
     View<const T> X;  // RHS input state (re-read in the epilogue)
     double c0, c1, c2, c3;
+    // Device time step: when dtp is set, the stage's dt-proportional
+    // coefficient (c2 of EPI_COMB, c1 otherwise) holds a multiplier m and
+    // becomes (m * dt) / dtq with dt = *dtp, the host's arithmetic.
+    const double *dtp;
+    double dtq;
 };
 
 template <class T> struct VolGeom {
@@ -124,6 +129,10 @@ hipError_t launch_wave_speed_derived(int order, int64_t E, const double *H, cons
                                      const double *v, int64_t es, double *partial,
                                      int64_t npart, hipStream_t s);
 hipError_t launch_max_reduce(const double *partial, int64_t npart, double *out, hipStream_t s);
+// AdaptiveTimeController::compute_dt of the device maximum *cmax into *dt,
+// and *t += dt when t is not null
+hipError_t launch_adaptive_dt(const double *cmax, double dx_min, double cfl, double dt_min,
+                              double dt_max, double *dt, double *t, hipStream_t s);
 
 // Mode split (modesplit.hip)
 hipError_t launch_mode_split(int order, int64_t E, double dt3, int nsub, double *full,

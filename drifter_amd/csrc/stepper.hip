@@ -100,7 +100,33 @@ __global__ void __launch_bounds__(256) max_reduce_kernel(const double *p, int64_
         out[0] = wave_max(wave_max(red[0], red[1]), wave_max(red[2], red[3]));
 }
 
+// AdaptiveTimeController::compute_dt (src/solver/time_stepper.cpp:261-268)
+// on the device maximum, and the driver's time_ += dt_
+// (simulation_driver.cpp:130).  One thread.
+__global__ void adaptive_dt_kernel(const double *__restrict__ cmax, double dx_min, double cfl,
+                                   double dt_min, double dt_max, double *__restrict__ dt,
+                                   double *__restrict__ t)
+{
+    const double c = *cmax;
+    double d = dt_max; // stationary: the largest step
+    if (!(c < 1e-14)) {
+        d = cfl * dx_min / c;
+        d = (d < dt_min) ? dt_min : ((dt_max < d) ? dt_max : d);
+    }
+    *dt = d;
+    if (t)
+        *t = *t + d;
+}
+
 } // namespace
+
+hipError_t launch_adaptive_dt(const double *cmax, double dx_min, double cfl, double dt_min,
+                              double dt_max, double *dt, double *t, hipStream_t s)
+{
+    hipLaunchKernelGGL(adaptive_dt_kernel, dim3(1), dim3(1), 0, s, cmax, dx_min, cfl, dt_min,
+                       dt_max, dt, t);
+    return hipGetLastError();
+}
 
 hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, View<const double> h,
                              double *partial, int64_t npart, hipStream_t s)

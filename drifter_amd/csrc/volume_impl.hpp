@@ -153,7 +153,15 @@ template <int n, int F0, int NFD, class TS, class TC, int EPI>
 __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int L,
                                              const TC (&kv)[NFD][n])
 {
-    const TC c0 = (TC)epi.c0, c1 = (TC)epi.c1, c2 = (TC)epi.c2, c3 = (TC)epi.c3;
+    double d1 = epi.c1, d2 = epi.c2;
+    if (EPI != EPI_RHS && epi.dtp) { // dt from device memory (adaptive stepping)
+        const double dt = *epi.dtp;
+        if (EPI == EPI_COMB)
+            d2 = (d2 * dt) / epi.dtq;
+        else
+            d1 = (d1 * dt) / epi.dtq;
+    }
+    const TC c0 = (TC)epi.c0, c1 = (TC)d1, c2 = (TC)d2, c3 = (TC)epi.c3;
     if (EPI == EPI_RHS) {
         const uint32_t ro = lane_off(epi.R, e, L);
 #pragma unroll

@@ -146,6 +146,26 @@ class PrimitiveEquationsSolver:
                                    C.byref(h)))
         return StepGraph(h)
 
+    def rk_step_adaptive_dev(self, scheme, U, dt_dev, time_dev=None, cfl=0.5, dt_min=1e-6,
+                             dt_max=1.0, dx_min=None, layout=_lib.LAYOUT_SOA, stream=None):
+        """SimulationDriver::step on the device: dt from the wave speed of U
+        into dt_dev (device float64, one value), time_dev += dt, one fused
+        step.  dx_min defaults to the driver's 2 / (N + 1)."""
+        dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
+        check(self._l.drf_rk_step_adaptive_dev(self._ctx, scheme, cfl, dx, dt_min, dt_max,
+                                               _ptr(U), layout, _ptr(dt_dev), _ptr(time_dev),
+                                               _sptr(stream)))
+
+    def capture_adaptive_steps(self, scheme, U, dt_dev, time_dev=None, n_steps=1, cfl=0.5,
+                               dt_min=1e-6, dt_max=1.0, dx_min=None, layout=_lib.LAYOUT_SOA):
+        """StepGraph of n_steps rk_step_adaptive_dev calls on U (in place)."""
+        dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
+        h = C.c_void_p()
+        check(self._l.drf_graph_rk_adaptive(self._ctx, scheme, cfl, dx, dt_min, dt_max, _ptr(U),
+                                            layout, n_steps, _ptr(dt_dev), _ptr(time_dev),
+                                            C.byref(h)))
+        return StepGraph(h)
+
     def max_wave_speed(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         c = C.c_double()

@@ -167,6 +167,26 @@ typedef struct drf_graph drf_graph;
  * coefficients, so both must outlive it.  Nothing is executed here. */
 drf_status drf_graph_rk(drf_ctx *ctx, int scheme, double dt, void *U, int layout, int fused,
                         int n_steps, drf_graph **out);
+
+/* SimulationDriver::step (simulation_driver.cpp:108-131) without a host
+ * round trip: the maximum wave speed of U (compute_max_wave_speed, :153-168),
+ * dt = AdaptiveTimeController::compute_dt(c, dx_min) (time_stepper.cpp:261-268)
+ * written to dt_dev, time_dev += dt (time_ += dt_, :130; time_dev may be
+ * NULL), then one fused step of `scheme` with that dt, which the stage
+ * kernels read from dt_dev.  dt_dev, time_dev: device doubles.  FP64
+ * contexts; fused schemes (not DP45).  Results equal drf_max_wave_speed_dev +
+ * drf_compute_dt + drf_rk_step_dev(fused) bit for bit.  dx_min is the
+ * driver's 2 / (N + 1) (:40-42). */
+drf_status drf_rk_step_adaptive_dev(drf_ctx *ctx, int scheme, double cfl, double dx_min,
+                                    double dt_min, double dt_max, void *U, int layout,
+                                    void *dt_dev, void *time_dev, void *stream);
+/* n_steps such steps captured as one graph (drf_graph_launch replays them;
+ * the host reads time_dev once per replay to decide whether to go on, as
+ * SimulationDriver::run_for's loop, :94-106).  The graph uses the context's
+ * reduction buffer: do not run other calls on the context concurrently. */
+drf_status drf_graph_rk_adaptive(drf_ctx *ctx, int scheme, double cfl, double dx_min,
+                                 double dt_min, double dt_max, void *U, int layout, int n_steps,
+                                 void *dt_dev, void *time_dev, drf_graph **out);
 /* Same for n_steps mode-split steps (drf_mode_split_step_dev): with the
  * barotropic subcycles inside each step's kernel, one launch replays
  * n_steps x n_sub substeps. */
