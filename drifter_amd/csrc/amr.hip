@@ -24,13 +24,13 @@ namespace drf {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kCopyChunk = 64; // elements per chunk of the ranked copy
+constexpr int kCopyChunk = 16; // elements per chunk of the ranked copy (many workgroups)
 
 template <int N>
 struct Shape {
     static constexpr int N2 = N * N, ND = N2 * N;
     static constexpr int EPB = ND >= 1024 ? 1 : 1024 / ND;       // outputs per batch (boxes)
-    static constexpr int PPB = 8 * ND >= 1024 ? 1 : 1024 / (8 * ND); // parents per batch
+    static constexpr int PPB = 8 * ND >= 2048 ? 1 : 2048 / (8 * ND); // parents per batch
 };
 
 __device__ __forceinline__ int64_t parent_source(BoxMode mode, int64_t p,
@@ -49,8 +49,8 @@ __global__ void __launch_bounds__(kThreads)
 {
     using S = Shape<N>;
     constexpr int N2 = S::N2, ND = S::ND, PPB = S::PPB;
+    constexpr int VPT = (PPB * ND + kThreads - 1) / kThreads; // parent values per thread
     __shared__ double sM[3 * N2], sU[PPB * ND], sT1[PPB * 2 * ND], sT2[PPB * 4 * ND];
-    __shared__ int64_t sSrc[PPB];
     const int t = threadIdx.x;
     const int rx = mask & 1, ry = (mask >> 1) & 1, rz = (mask >> 2) & 1;
     const int nx = 1 + rx, ny = 1 + ry, nz = 1 + rz, nxy = nx * ny, nch = nxy * nz;
@@ -59,36 +59,55 @@ __global__ void __launch_bounds__(kThreads)
         sM[q] = ns.M[b][(r / N) * kMaxN + r % N];
     }
     const int64_t nb = (n_parent + PPB - 1) / PPB;
-    for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    // the parents' values of batch b, loaded ahead into registers
+    double nxt[VPT];
+    auto fetch = [&](int64_t b) {
         const int64_t p0 = b * PPB;
         const int cnt = (n_parent - p0 < PPB) ? (int)(n_parent - p0) : PPB;
-        if (t < cnt) {
-            const int64_t src = parent_source(MODE, p0 + t, idx);
-            sSrc[t] = (src >= 0 && src < n_src) ? src : -1;
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) {
+            const int q = t + v * kThreads;
+            nxt[v] = 0.0;
+            if (q < cnt * ND) {
+                const int e = q / ND, d = q - e * ND;
+                const int64_t src = parent_source(MODE, p0 + e, idx);
+                nxt[v] = (src >= 0 && src < n_src) ? U[src * es + d] : __builtin_nan("");
+            }
         }
-        __syncthreads();
-        for (int q = t; q < cnt * ND; q += kThreads) {
-            const int e = q / ND, d = q - e * ND;
-            const int64_t src = sSrc[e];
-            sU[q] = (src >= 0) ? U[src * es + d] : __builtin_nan("");
+    };
+    int64_t b = blockIdx.x;
+    if (b < nb)
+        fetch(b);
+    for (; b < nb; b += gridDim.x) {
+        const int64_t p0 = b * PPB;
+        const int cnt = (n_parent - p0 < PPB) ? (int)(n_parent - p0) : PPB;
+        // sU was last read before two barriers of the previous batch
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) {
+            const int q = t + v * kThreads;
+            if (q < cnt * ND)
+                sU[q] = nxt[v];
         }
+        if (b + gridDim.x < nb)
+            fetch(b + gridDim.x);
         __syncthreads();
         // xi: T1[e][hx][k][j][p] = sum_i Mx_hx[p][i] u[e][k][j][i]
+        // nx, ny, nz are 1 or 2: unit u = q / ND splits by shifts and masks
         for (int q = t; q < cnt * nx * ND; q += kThreads) {
-            const int e = q / (nx * ND), r = q - e * nx * ND, hx = r / ND, d = r - hx * ND;
+            const int u = q / ND, d = q - u * ND, e = u >> rx, hx = u & rx;
             const int line = d / N, p = d - line * N;
-            const double *M = sM + (rx ? 1 + hx : 0) * N2 + p * N, *u = sU + e * ND + line * N;
+            const double *M = sM + (rx ? 1 + hx : 0) * N2 + p * N, *x = sU + e * ND + line * N;
             double s = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i)
-                s += M[i] * u[i];
+                s += M[i] * x[i];
             sT1[q] = s;
         }
         __syncthreads();
         // eta: T2[e][hy][hx][k][q][p] = sum_j My_hy[q][j] T1[e][hx][k][j][p]
         for (int q = t; q < cnt * nxy * ND; q += kThreads) {
-            const int e = q / (nxy * ND), r = q - e * nxy * ND, hyx = r / ND, d = r - hyx * ND;
-            const int hy = hyx / nx, hx = hyx - hy * nx;
+            const int u = q / ND, d = q - u * ND, e = u >> (rx + ry), hyx = u & (nxy - 1);
+            const int hy = hyx >> rx, hx = hyx & rx;
             const int k = d / N2, rr = d - k * N2, qq = rr / N, p = rr - qq * N;
             const double *M = sM + (ry ? 1 + hy : 0) * N2 + qq * N;
             const double *T = sT1 + (e * nx + hx) * ND + k * N2 + p;
@@ -101,8 +120,8 @@ __global__ void __launch_bounds__(kThreads)
         __syncthreads();
         // zeta: child c = hx + nx (hy + ny hz): out[r][L] = sum_k Mz_hz[r][k] T2[e][hy][hx][k][L]
         for (int q = t; q < cnt * nch * ND; q += kThreads) {
-            const int e = q / (nch * ND), r = q - e * nch * ND, c = r / ND, d = r - c * ND;
-            const int hz = c / nxy, hyx = c - hz * nxy;
+            const int u = q / ND, d = q - u * ND, e = u >> (rx + ry + rz), c = u & (nch - 1);
+            const int hz = c >> (rx + ry), hyx = c & (nxy - 1);
             const int rr = d / N2, L = d - rr * N2;
             const double *M = sM + (rz ? 1 + hz : 0) * N2 + rr * N;
             const double *T = sT2 + (e * nxy + hyx) * ND + L;
@@ -112,7 +131,8 @@ __global__ void __launch_bounds__(kThreads)
                 s += M[k] * T[k * N2];
             out[(base + (p0 + e) * nch + c) * os + d] = s;
         }
-        __syncthreads();
+        // no barrier: the next batch writes sU (read only before the eta
+        // pass's barrier), then waits before touching sT1 and sT2
     }
 }
 
@@ -295,7 +315,8 @@ __global__ void __launch_bounds__(kThreads)
     __shared__ double sX[EPB * 4 * ND], sY[EPB * 2 * ND], sQ[3 * N2];
     __shared__ int sH[3 * N];
     const int t = threadIdx.x;
-    const int nx = (mask & 1) ? 2 : 1, ny = (mask & 2) ? 2 : 1, nz = (mask & 4) ? 2 : 1;
+    const int rx = mask & 1, ry = (mask >> 1) & 1, rz = (mask >> 2) & 1;
+    const int nx = 1 + rx, ny = 1 + ry, nz = 1 + rz;
     const int nch = nx * ny * nz, nzy = nz * ny;
     for (int q = t; q < 3 * N2; q += kThreads) {
         const int a = q / N2, r = q - a * N2;
@@ -309,9 +330,9 @@ __global__ void __launch_bounds__(kThreads)
         const int64_t p0 = b * EPB;
         const int cnt = (n_parent - p0 < EPB) ? (int)(n_parent - p0) : EPB;
         for (int q = t; q < cnt * nzy * ND; q += kThreads) {
-            const int e = q / (nzy * ND), r = q - e * nzy * ND, czy = r / ND, d = r - czy * ND;
+            const int u = q / ND, d = q - u * ND, e = u >> (ry + rz), czy = u & (nzy - 1);
             const int line = d / N, pi = d - line * N;
-            const int cz = czy / ny, cy = czy - cz * ny;
+            const int cz = czy >> ry, cy = czy & ry;
             const int c = (cz * ny + cy) * nx + sH[pi];
             const double *src = C + ((p0 + e) * nch + c) * cs + line * N;
             const double *Q = sQ + pi * N;
@@ -323,7 +344,7 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
         for (int q = t; q < cnt * nz * ND; q += kThreads) {
-            const int e = q / (nz * ND), r = q - e * nz * ND, cz = r / ND, d = r - cz * ND;
+            const int u = q / ND, d = q - u * ND, e = u >> rz, cz = u & rz;
             const int k = d / N2, rr = d - k * N2, pj = rr / N, pi = rr - pj * N;
             const double *X = sX + e * nzy * ND + (cz * ny + sH[N + pj]) * ND + k * N2 + pi;
             const double *Q = sQ + N2 + pj * N;
