@@ -103,6 +103,14 @@ SIGNATURES = {
     "drf_mlp_loss_grad_dev": (_I, [_P, _L, _P, _P, _P, _P, _P]),
     "drf_mlp_train_step_dev": (_I, [_P, _L, _P, _P, _D, _P, _P]),
     "drf_sigma_metrics_dev": (_I, [_L] + [_P] * 14),
+    "drf_diagnose_omega_dev": (_I, [_I, _L, _P, _P, _L, _P, _P]),
+    "drf_diagnose_omega_from_velocity_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P]),
+    "drf_omega_to_w_dev": (_I, [_L] + [_P] * 9),
+    "drf_omega_to_w_simple_dev": (_I, [_L] + [_P] * 11),
+    "drf_surface_omega": (_D, [_D] * 6),
+    "drf_column_sigma_nodes": (_I, [_I, _PD]),
+    "drf_diagnose_column_dev": (_I, [_I, _L, _P, _P, _P, _P, _P]),
+    "drf_continuity_residual_dev": (_I, [_I, _L, _P, _P, _L, _P, _P]),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

@@ -463,6 +463,50 @@ drf_status drf_sigma_metrics_dev(int64_t npts, const void *sigma, const void *et
                                  void *dsigma_dt, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Vertical velocity diagnosis (VerticalVelocityDiagnosis,             */
+/* ColumnVerticalVelocity, src/physics/vertical_velocity.cpp); FP64,   */
+/* not used by the right-hand side (omega = 0 there)                   */
+/* ------------------------------------------------------------------ */
+
+/* diagnose_omega (:55-97) for n_elem elements (element e at + e*elem_stride,
+ * device): per column of N+1 consecutive DOFs (node_index_3d), omega = 0 at
+ * the first, then -(trapezoid integral of div over the column, dsigma =
+ * dzeta / 2) / mean H of the segment, 0 where that mean H <= 1e-10. */
+drf_status drf_diagnose_omega_dev(int order, int64_t n_elem, const void *div, const void *H,
+                                  int64_t elem_stride, void *omega, void *stream);
+/* diagnose_omega_from_velocity (:99-108): div = dHu_dx + dHv_dy, then as
+ * above (the reference's Hu and Hv arguments are unused there) */
+drf_status drf_diagnose_omega_from_velocity_dev(int order, int64_t n_elem, const void *dHu_dx,
+                                                const void *dHv_dy, const void *H,
+                                                int64_t elem_stride, void *omega, void *stream);
+/* omega_to_physical_w (:110-121): w = H omega + u dz_dx + v dz_dy + dz_dt per point */
+drf_status drf_omega_to_w_dev(int64_t npts, const void *omega, const void *u, const void *v,
+                              const void *H, const void *dz_dx, const void *dz_dy,
+                              const void *dz_dt, void *w, void *stream);
+/* omega_to_w_simple (:123-143): the same with H = eta + h and dz/d(x, y, t) =
+ * deta/d(x, y, t) (1 + sigma) */
+drf_status drf_omega_to_w_simple_dev(int64_t npts, const void *omega, const void *u,
+                                     const void *v, const void *eta, const void *h,
+                                     const void *sigma, const void *deta_dx,
+                                     const void *deta_dy, const void *deta_dt, void *w,
+                                     void *stream);
+/* surface_omega (:145-152), host: (deta_dt + u deta_dx + v deta_dy) / H, 0
+ * when H < 1e-10.  (bottom_omega, :154-159, is always 0.) */
+double drf_surface_omega(double deta_dt, double u_surf, double v_surf, double deta_dx,
+                         double deta_dy, double H);
+/* ColumnVerticalVelocity(order) sigma nodes (:180-244), host, order+1 values:
+ * LGL nodes mapped to [-1, 0] for 3..5 levels, uniform spacing above */
+drf_status drf_column_sigma_nodes(int order, double *sigma);
+/* diagnose_column (:246-275) for ncol columns of order+1 contiguous levels;
+ * omega_bottom: device double per column, or NULL for 0 */
+drf_status drf_diagnose_column_dev(int order, int64_t ncol, const void *div, const void *H,
+                                   const void *omega_bottom, void *omega, void *stream);
+/* continuity_residual (:161-174) per element: sqrt(sum (div + dHomega)^2 / ndof) */
+drf_status drf_continuity_residual_dev(int order, int64_t n_elem, const void *div,
+                                       const void *dHomega_dsigma, int64_t elem_stride,
+                                       void *residual, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */
 /* ------------------------------------------------------------------ */
 /* Algorithmic HBM bytes of one drf_rhs_dev launch at the current mesh. */

@@ -180,6 +180,25 @@ int orc_project_from_children(int order, int mask, const double *children, doubl
 int64_t orc_transfer_solution(int order, int64_t E, const double *U, int64_t n_ref,
                               const int64_t *parents, double *U_new);
 
+/* ---- vertical velocity diagnosis (vertical_velocity.cpp) ---- */
+int orc_diagnose_omega(int order, int64_t E, const double *div, const double *H, int64_t es,
+                       double *omega);
+int orc_diagnose_omega_from_velocity(int order, int64_t E, const double *dHu_dx,
+                                     const double *dHv_dy, const double *H, int64_t es,
+                                     double *omega);
+void orc_omega_to_w(int64_t npts, const double *omega, const double *u, const double *v,
+                    const double *H, const double *dz_dx, const double *dz_dy,
+                    const double *dz_dt, double *w);
+void orc_omega_to_w_simple(int64_t npts, const double *omega, const double *u, const double *v,
+                           const double *eta, const double *h, const double *sigma,
+                           const double *deta_dx, const double *deta_dy, const double *deta_dt,
+                           double *w);
+int orc_column_sigma_nodes(int order, double *sigma);
+int orc_diagnose_column(int order, int64_t ncol, const double *div, const double *H,
+                        const double *omega_bottom, double *omega);
+void orc_continuity_residual(int order, int64_t E, const double *div, const double *dHomega,
+                             int64_t es, double *out);
+
 #ifdef __cplusplus
 }
 #endif

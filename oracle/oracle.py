@@ -75,6 +75,14 @@ def load():
         "orc_sigma_metrics": (None, [C.c_int64] + [_PD] * 13),
         "orc_bf16_round": (C.c_double, [C.c_double]),
         "orc_project_to_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
+        "orc_diagnose_omega": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, C.c_int64, _PD]),
+        "orc_diagnose_omega_from_velocity": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, _PD,
+                                                       C.c_int64, _PD]),
+        "orc_omega_to_w": (None, [C.c_int64] + [_PD] * 8),
+        "orc_omega_to_w_simple": (None, [C.c_int64] + [_PD] * 10),
+        "orc_column_sigma_nodes": (C.c_int, [C.c_int, _PD]),
+        "orc_diagnose_column": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, C.c_void_p, _PD]),
+        "orc_continuity_residual": (None, [C.c_int, C.c_int64, _PD, _PD, C.c_int64, _PD]),
         "orc_project_from_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
         "orc_transfer_solution": (C.c_int64, [C.c_int, C.c_int64, _PD, C.c_int64, C.c_void_p,
                                               _PD]),
@@ -375,3 +383,57 @@ def transfer_solution(order, U, parents):
     out = np.zeros((E + 8 * len(parents), nd))
     k = load().orc_transfer_solution(order, E, _p(U), len(parents), parents.ctypes.data, _p(out))
     return out[:k]
+
+
+def diagnose_omega(order, div, H):
+    """div, H (E, ndof) -> omega (E, ndof)"""
+    div, H = _v(div), _v(H)
+    out = np.zeros_like(div)
+    load().orc_diagnose_omega(order, div.shape[0], _p(div), _p(H), div.shape[1], _p(out))
+    return out
+
+
+def diagnose_omega_from_velocity(order, dHu_dx, dHv_dy, H):
+    a, b, H = _v(dHu_dx), _v(dHv_dy), _v(H)
+    out = np.zeros_like(a)
+    load().orc_diagnose_omega_from_velocity(order, a.shape[0], _p(a), _p(b), _p(H), a.shape[1],
+                                            _p(out))
+    return out
+
+
+def omega_to_w(omega, u, v, H, dz_dx, dz_dy, dz_dt):
+    a = [_v(x).reshape(-1) for x in (omega, u, v, H, dz_dx, dz_dy, dz_dt)]
+    out = np.zeros(a[0].size)
+    load().orc_omega_to_w(a[0].size, *[_p(x) for x in a], _p(out))
+    return out
+
+
+def omega_to_w_simple(omega, u, v, eta, h, sigma, deta_dx, deta_dy, deta_dt):
+    a = [_v(x).reshape(-1) for x in (omega, u, v, eta, h, sigma, deta_dx, deta_dy, deta_dt)]
+    out = np.zeros(a[0].size)
+    load().orc_omega_to_w_simple(a[0].size, *[_p(x) for x in a], _p(out))
+    return out
+
+
+def column_sigma_nodes(order):
+    out = np.zeros(order + 1)
+    load().orc_column_sigma_nodes(order, _p(out))
+    return out
+
+
+def diagnose_column(order, div, H, omega_bottom=None):
+    """div, H (ncol, order+1) -> omega (ncol, order+1)"""
+    div, H = _v(div), _v(H)
+    ob = None if omega_bottom is None else np.ascontiguousarray(omega_bottom, dtype=np.float64)
+    out = np.zeros_like(div)
+    load().orc_diagnose_column(order, div.shape[0], _p(div), _p(H),
+                               None if ob is None else ob.ctypes.data, _p(out))
+    return out
+
+
+def continuity_residual(order, div, dHomega):
+    """div, dHomega (E, ndof) -> RMS residual per element (E,)"""
+    div, dh = _v(div), _v(dHomega)
+    out = np.zeros(div.shape[0])
+    load().orc_continuity_residual(order, div.shape[0], _p(div), _p(dh), div.shape[1], _p(out))
+    return out

@@ -32,6 +32,32 @@ def test_every_declared_symbol_is_exported():
     assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
 
 
+def _declared_arity():
+    """Parameter count of every declaration in the headers."""
+    out = {}
+    for hdr in os.listdir(os.path.join(ROOT, "include", "drifter_hip")):
+        if not hdr.endswith(".h"):
+            continue
+        txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "drifter_hip",
+                                                        hdr)).read(), flags=re.S)
+        decl = (r"^\s*(?:const\s+)?(?:drf_status|void|double|int64_t|int|char)\s*\*?\s*"
+                r"(drf_\w+)\s*\(([^)]*)\)\s*;")
+        for name, params in re.findall(decl, txt, flags=re.M):
+            p = params.strip()
+            out[name] = 0 if p in ("", "void") else p.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_the_header_arity():
+    """Every ctypes argument list has as many entries as the C declaration
+    (a short list would pass garbage for the last arguments)."""
+    ar = _declared_arity()
+    assert len(ar) >= 20
+    bad = {n: (k, len(_lib.SIGNATURES[n][1])) for n, k in ar.items()
+           if n in _lib.SIGNATURES and len(_lib.SIGNATURES[n][1]) != k}
+    assert not bad, bad
+
+
 def test_invalid_order_is_rejected_before_touching_a_device():
     lib = _lib.load()
     h = C.c_void_p()
