@@ -103,6 +103,12 @@ SIGNATURES = {
     "drf_mlp_loss_grad_dev": (_I, [_P, _L, _P, _P, _P, _P, _P]),
     "drf_mlp_train_step_dev": (_I, [_P, _L, _P, _P, _D, _P, _P]),
     "drf_sigma_metrics_dev": (_I, [_L] + [_P] * 14),
+    "drf_sigma_ops_metrics_dev": (_I, [_I, _L] + [_P] * 7 + [_L] + [_P] * 7),
+    "drf_sigma_horizontal_gradient_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P, _P]),
+    "drf_sigma_vertical_gradient_dev": (_I, [_I, _L, _P, _P, _L, _P, _P]),
+    "drf_sigma_divergence_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P]),
+    "drf_sigma_ale_correction_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P]),
+    "drf_sigma_material_derivative_dev": (_I, [_I, _L] + [_P] * 9 + [_L, _P, _P]),
     "drf_diagnose_omega_dev": (_I, [_I, _L, _P, _P, _L, _P, _P]),
     "drf_diagnose_omega_from_velocity_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P]),
     "drf_omega_to_w_dev": (_I, [_L] + [_P] * 9),

@@ -463,6 +463,51 @@ drf_status drf_sigma_metrics_dev(int64_t npts, const void *sigma, const void *et
                                  void *dsigma_dt, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* SigmaCoordinateOperators (src/dg/operators_3d.cpp:513-662); FP64,   */
+/* opt-in, not used by the right-hand side                             */
+/* ------------------------------------------------------------------ */
+/* All fields one nodal set per element (element e at + e*elem_stride,
+ * device), the same stride for every field of a call.  Derivatives are the
+ * reference-coordinate D_xi, D_eta, D_zeta (the reference's dense Kronecker
+ * products, applied line by line in the same order). */
+
+/* update_sigma_metrics_with_time (:578-615); deta_dt NULL is
+ * update_sigma_metrics (:569-576).  Per node: H = eta + h, H_inv (0 when
+ * H <= 1e-10), sigma = (zeta - 1) / 2 of the node, dsigma_dx = (sigma dH/dx -
+ * deta/dx) H_inv (this class's sign; SigmaCoordinate::dsigma_dx has
+ * -(deta/dx + sigma dH/dx) / H), dsigma_dy likewise, dsigma_dt = -(1 + sigma)
+ * deta/dt H_inv. */
+drf_status drf_sigma_ops_metrics_dev(int order, int64_t n_elem, const void *eta, const void *h,
+                                     const void *deta_dx, const void *deta_dy, const void *dh_dx,
+                                     const void *dh_dy, const void *deta_dt, int64_t elem_stride,
+                                     void *H, void *H_inv, void *sigma, void *dsigma_dx,
+                                     void *dsigma_dy, void *dsigma_dt, void *stream);
+/* horizontal_gradient (:517-542): dU_dx = D_xi U - dsigma_dx D_zeta U, dU_dy likewise */
+drf_status drf_sigma_horizontal_gradient_dev(int order, int64_t n_elem, const void *U,
+                                             const void *dsigma_dx, const void *dsigma_dy,
+                                             int64_t elem_stride, void *dU_dx, void *dU_dy,
+                                             void *stream);
+/* vertical_gradient (:544-554): dU_dz = H_inv D_zeta U */
+drf_status drf_sigma_vertical_gradient_dev(int order, int64_t n_elem, const void *U,
+                                           const void *H_inv, int64_t elem_stride, void *dU_dz,
+                                           void *stream);
+/* sigma_divergence (:556-567): D_xi Hu + D_eta Hv + D_zeta omega */
+drf_status drf_sigma_divergence_dev(int order, int64_t n_elem, const void *Hu, const void *Hv,
+                                    const void *omega, int64_t elem_stride, void *div,
+                                    void *stream);
+/* ale_correction (:617-632): -w_mesh H_inv D_zeta U */
+drf_status drf_sigma_ale_correction_dev(int order, int64_t n_elem, const void *U,
+                                        const void *w_mesh, const void *H_inv,
+                                        int64_t elem_stride, void *correction, void *stream);
+/* material_derivative (:634-662): dU_dt + u dU_dx + v dU_dy + (omega - w_mesh H_inv) D_zeta U */
+drf_status drf_sigma_material_derivative_dev(int order, int64_t n_elem, const void *U,
+                                             const void *dU_dt, const void *u, const void *v,
+                                             const void *omega, const void *w_mesh,
+                                             const void *dsigma_dx, const void *dsigma_dy,
+                                             const void *H_inv, int64_t elem_stride, void *out,
+                                             void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Vertical velocity diagnosis (VerticalVelocityDiagnosis,             */
 /* ColumnVerticalVelocity, src/physics/vertical_velocity.cpp); FP64,   */
 /* not used by the right-hand side (omega = 0 there)                   */

@@ -199,6 +199,16 @@ int orc_diagnose_column(int order, int64_t ncol, const double *div, const double
 void orc_continuity_residual(int order, int64_t E, const double *div, const double *dHomega,
                              int64_t es, double *out);
 
+/* ---- SigmaCoordinateOperators (operators_3d.cpp:513-662) ---- */
+int orc_sigma_ops_metrics(int order, int64_t E, const double *eta, const double *h,
+                          const double *deta_dx, const double *deta_dy, const double *dh_dx,
+                          const double *dh_dy, const double *deta_dt, int64_t es, double *H,
+                          double *Hinv, double *sigma, double *dsdx, double *dsdy, double *dsdt);
+int orc_sigma_ops_apply(int order, int op, int64_t E, const double *U, const double *a,
+                        const double *b, const double *c, const double *f, const double *g,
+                        const double *dsdx, const double *dsdy, const double *Hinv, int64_t es,
+                        double *out1, double *out2);
+
 #ifdef __cplusplus
 }
 #endif

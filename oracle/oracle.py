@@ -83,6 +83,10 @@ def load():
         "orc_column_sigma_nodes": (C.c_int, [C.c_int, _PD]),
         "orc_diagnose_column": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, C.c_void_p, _PD]),
         "orc_continuity_residual": (None, [C.c_int, C.c_int64, _PD, _PD, C.c_int64, _PD]),
+        "orc_sigma_ops_metrics": (C.c_int, [C.c_int, C.c_int64] + [C.c_void_p] * 7 +
+                                  [C.c_int64] + [_PD] * 6),
+        "orc_sigma_ops_apply": (C.c_int, [C.c_int, C.c_int, C.c_int64] + [C.c_void_p] * 9 +
+                                [C.c_int64, C.c_void_p, C.c_void_p]),
         "orc_project_from_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
         "orc_transfer_solution": (C.c_int64, [C.c_int, C.c_int64, _PD, C.c_int64, C.c_void_p,
                                               _PD]),
@@ -437,3 +441,34 @@ def continuity_residual(order, div, dHomega):
     out = np.zeros(div.shape[0])
     load().orc_continuity_residual(order, div.shape[0], _p(div), _p(dh), div.shape[1], _p(out))
     return out
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data
+
+
+def sigma_ops_metrics(order, eta, h, deta_dx, deta_dy, dh_dx, dh_dy, deta_dt=None):
+    """(E, ndof) arrays -> dict H, Hinv, sigma, dsdx, dsdy, dsdt"""
+    ins = [_v(a) for a in (eta, h, deta_dx, deta_dy, dh_dx, dh_dy)]
+    dt = None if deta_dt is None else _v(deta_dt)
+    E, nd = ins[0].shape
+    outs = {k: np.zeros((E, nd)) for k in ("H", "Hinv", "sigma", "dsdx", "dsdy", "dsdt")}
+    load().orc_sigma_ops_metrics(order, E, *[_vp(a) for a in ins], _vp(dt), nd,
+                                 *[_p(outs[k]) for k in ("H", "Hinv", "sigma", "dsdx", "dsdy",
+                                                         "dsdt")])
+    return outs
+
+
+SIGMA_OP_HGRAD, SIGMA_OP_VGRAD, SIGMA_OP_DIV, SIGMA_OP_ALE, SIGMA_OP_MATDER = range(5)
+
+
+def sigma_ops_apply(order, op, U, a=None, b=None, c=None, f=None, g=None, dsdx=None, dsdy=None,
+                    Hinv=None):
+    U = _v(U)
+    E, nd = U.shape
+    conv = [None if x is None else _v(x) for x in (a, b, c, f, g, dsdx, dsdy, Hinv)]
+    o1 = np.zeros((E, nd))
+    o2 = np.zeros((E, nd)) if op == SIGMA_OP_HGRAD else None
+    load().orc_sigma_ops_apply(order, op, E, _vp(U), *[_vp(x) for x in conv], nd, _vp(o1),
+                               _vp(o2))
+    return (o1, o2) if op == SIGMA_OP_HGRAD else o1
