@@ -152,6 +152,43 @@ drf_status drf_transfer_solution_dev(int order, int64_t n_elem, const void *U, i
     return DRF_OK;
 }
 
+drf_status drf_transfer_solution(int order, int64_t n_elem, const double *U, int64_t n_refine,
+                                 const int64_t *parents, double *U_new, int device)
+{
+    DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_UNSUPPORTED,
+                "drf_transfer_solution: order out of range");
+    DRF_REQUIRE(n_elem >= 0 && n_refine >= 0 && n_refine <= n_elem, DRF_ERR_ARG,
+                "drf_transfer_solution: bad sizes");
+    const int64_t nd = (int64_t)(order + 1) * (order + 1) * (order + 1);
+    const int64_t n_out = n_elem + 7 * n_refine;
+    if (n_out == 0)
+        return DRF_OK;
+    DRF_REQUIRE(U && U_new && (n_refine == 0 || parents), DRF_ERR_ARG,
+                "drf_transfer_solution: null pointer");
+    for (int64_t r = 0; r < n_refine; ++r)
+        DRF_REQUIRE(parents[r] >= 0 && parents[r] < n_elem && (r == 0 || parents[r - 1] < parents[r]),
+                    DRF_ERR_ARG,
+                    "drf_transfer_solution: parents must be strictly ascending element indices");
+    DRF_HIP_CHECK(hipSetDevice(device));
+    // one allocation per call: the adapt cycle is rare next to the RHS
+    struct Bufs {
+        void *p = nullptr;
+        ~Bufs() { (void)hipFree(p); }
+    } b;
+    const size_t bu = sizeof(double) * n_elem * nd, bo = sizeof(double) * n_out * nd;
+    DRF_HIP_CHECK(hipMalloc(&b.p, bu + bo + sizeof(int64_t) * (n_refine + 1)));
+    double *dU = (double *)b.p, *dN = (double *)((char *)b.p + bu);
+    int64_t *dP = (int64_t *)((char *)b.p + bu + bo);
+    DRF_HIP_CHECK(hipMemcpy(dU, U, bu, hipMemcpyHostToDevice));
+    if (n_refine)
+        DRF_HIP_CHECK(hipMemcpy(dP, parents, sizeof(int64_t) * n_refine, hipMemcpyHostToDevice));
+    drf_status st = drf_transfer_solution_dev(order, n_elem, dU, nd, n_refine, dP, dN, nd, nullptr);
+    if (st)
+        return st;
+    DRF_HIP_CHECK(hipMemcpy(U_new, dN, bo, hipMemcpyDeviceToHost));
+    return DRF_OK;
+}
+
 drf_status drf_interpolate_boxes_dev(int order, int64_t n_out, int64_t n_elem, const void *src,
                                      const void *boxes, const void *U, int64_t elem_stride,
                                      void *U_new, int64_t new_stride, void *stream)

@@ -373,6 +373,12 @@ drf_status drf_project_from_children_dev(int order, int mask, int64_t n_parent,
 drf_status drf_transfer_solution_dev(int order, int64_t n_elem, const void *U, int64_t elem_stride,
                                      int64_t n_refine, const void *parents, void *U_new,
                                      int64_t new_stride, void *stream);
+/* The same from host memory (the reference's vector of per-element VecX,
+ * packed: element e at U + e (N+1)^3), with parents checked on the host
+ * (DRF_ERR_ARG unless strictly ascending and < n_elem).  U_new receives
+ * n_elem + 7 n_refine elements.  Copies over PCIe; allocates per call. */
+drf_status drf_transfer_solution(int order, int64_t n_elem, const double *U, int64_t n_refine,
+                                 const int64_t *parents, double *U_new, int device);
 /* Transfer onto the refined mesh's own leaf order (not the reference's
  * behaviour: it fixes the always-XYZ projection and the appended-children
  * order of transfer_solution).  Output element k is the source element

@@ -8,15 +8,19 @@
 //   MortarSpace               include/dg/mortar.hpp:32-136
 //   ConformingInterface       include/dg/mortar.hpp:187-215
 //   GradientErrorEstimator    include/amr/refinement.hpp, src/amr/refinement.cpp:39-161
+//   transfer_solution         AdaptiveMeshRefinement::transfer_solution, src/amr/refinement.cpp:533-567
 // Vectors are std::vector<double> (Eigen's VecX maps onto them through
 // .data()); failures throw std::runtime_error carrying drf_last_error(), as
 // the reference reports errors by exception.
 #pragma once
 
+#include <algorithm>
+#include <cstdint>
 #include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "drf.h"
@@ -251,6 +255,35 @@ struct ConformingInterface {
 
 // GradientErrorEstimator(basis, params) + estimate + the marking of
 // AdaptiveMeshRefinement::mark_elements, on device buffers.
+// AdaptiveMeshRefinement::transfer_solution on the GPU.  Appends to
+// new_solution as the reference does (adapt() hands in the solution vector
+// itself, refinement.cpp:471): the elements that are not parents in order,
+// then the 8 children of an XYZ projection of each parent in map order.  The
+// parents must be ascending, as adapt() builds the map (:452-466).
+inline void transfer_solution(int order, const std::vector<Vec> &old_solution,
+                              const std::vector<std::pair<int64_t, std::vector<int64_t>>> &refinement_map,
+                              std::vector<Vec> &new_solution, int device = 0)
+{
+    const size_t nd = (size_t)(order + 1) * (order + 1) * (order + 1);
+    const int64_t E = (int64_t)old_solution.size();
+    Vec U(E * nd);
+    for (int64_t e = 0; e < E; ++e) {
+        if (old_solution[e].size() != nd)
+            throw Error(DRF_ERR_ARG, "transfer_solution: element vector of the wrong size");
+        std::copy(old_solution[e].begin(), old_solution[e].end(), U.begin() + e * nd);
+    }
+    std::vector<int64_t> parents;
+    parents.reserve(refinement_map.size());
+    for (const auto &entry : refinement_map)
+        parents.push_back(entry.first);
+    const int64_t R = (int64_t)parents.size(), n_out = E + 7 * R;
+    Vec out(n_out * nd);
+    check(drf_transfer_solution(order, E, U.data(), R, parents.data(), out.data(), device));
+    new_solution.reserve(new_solution.size() + n_out);
+    for (int64_t k = 0; k < n_out; ++k)
+        new_solution.emplace_back(out.begin() + k * nd, out.begin() + (k + 1) * nd);
+}
+
 class GradientErrorEstimator {
   public:
     GradientErrorEstimator(int order, Real refine_threshold = 0.1, Real coarsen_threshold = 0.01,

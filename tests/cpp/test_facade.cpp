@@ -140,6 +140,40 @@ int main()
             std::printf("mortar type %d: %d fine, |dM| %.1e |dPf| %.1e |dLf| %.1e %s\n", t, nf,
                         dM, dP, dL, ok ? "ok" : "MISMATCH");
         }
+        // AdaptiveMeshRefinement::transfer_solution against the oracle
+        {
+            const int to = 3, tnd = 64, tE = 20;
+            std::vector<Vec> old_sol(tE, Vec(tnd));
+            Vec packed(tE * tnd);
+            unsigned seed = 12345u;
+            for (int e = 0; e < tE; ++e)
+                for (int d = 0; d < tnd; ++d) {
+                    seed = seed * 1664525u + 1013904223u;
+                    old_sol[e][d] = packed[e * tnd + d] = (double)(seed >> 8) / 16777216.0 - 0.5;
+                }
+            std::vector<std::pair<int64_t, std::vector<int64_t>>> rmap;
+            const int64_t par[3] = {2, 5, 19};
+            for (int64_t p : par)
+                rmap.push_back({p, std::vector<int64_t>(8, -1)});
+            std::vector<Vec> got(1, Vec(tnd, 7.0)); // appended to, as the reference does
+            drifter_gpu::transfer_solution(to, old_sol, rmap, got);
+            Vec ref((tE + 21) * tnd);
+            orc_transfer_solution(to, tE, packed.data(), 3, par, ref.data());
+            double dmax = 0.0, scale = 0.0;
+            bool copies_exact = got.size() == (size_t)(1 + tE + 21) && got[0][0] == 7.0;
+            for (int k = 0; copies_exact && k < tE + 21; ++k)
+                for (int d = 0; d < tnd; ++d) {
+                    const double a = got[1 + k][d], b = ref[(size_t)k * tnd + d];
+                    if (k < tE - 3 && a != b)
+                        copies_exact = false;
+                    dmax = std::fmax(dmax, std::fabs(a - b));
+                    scale = std::fmax(scale, std::fabs(b));
+                }
+            const bool ok = copies_exact && dmax <= 1e-13 * scale;
+            fails += !ok;
+            std::printf("transfer_solution: %zu elements, max |diff| %.1e %s\n", got.size() - 1,
+                        dmax, ok ? "ok" : "MISMATCH");
+        }
     } catch (const std::exception &ex) {
         std::printf("FAIL: exception %s\n", ex.what());
         return 1;
