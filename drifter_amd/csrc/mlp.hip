@@ -48,9 +48,10 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 // 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)): one exp2 and one reciprocal.
 // The derivative is sigma + 2 z sigma (1 - sigma) u'(z).
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kG1 = -2.0f * kLog2e * kC, kG3 = kG1 * kA; // -2u log2(e) = z (kG1 + kG3 z^2)
 __device__ __forceinline__ float gelu_sigma(float z)
 {
-    const float e = __builtin_amdgcn_exp2f(-2.0f * kLog2e * kC * (z + kA * z * z * z));
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(kG3, z * z, kG1));
     return __builtin_amdgcn_rcpf(1.0f + e); // e = inf gives 0
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }

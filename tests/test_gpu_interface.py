@@ -165,3 +165,52 @@ def test_mortar_large_batch_stride(torch, orc):
         s = max(np.abs(ec).max(), np.abs(ef).max())
         np.testing.assert_allclose(rc[k], ec, rtol=0, atol=1e-12 * s)
         np.testing.assert_allclose(rf[k], ef, rtol=0, atol=1e-12 * s)
+
+
+@pytest.mark.parametrize("n_if", [0, 1, 15, 16, 17])
+def test_mortar_small_counts(torch, orc, n_if):
+    """counts around one wavefront's 16 interfaces (and none)"""
+    from drifter_amd import interface as itf
+
+    order, ctype, kind = 3, 5, 6
+    ms = itf.MortarSpace(order, ctype, device=0)
+    nm, nfine = ms.num_dofs(), ms.num_fine_elements()
+    rng = np.random.default_rng(n_if + 3)
+    Uc, Uf = rng.normal(size=(n_if, nm)), rng.normal(size=(n_if, nfine, nm))
+    nrm = _unit_normals(rng, max(n_if, 1), planar=False)[:n_if]
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    print(f"[mortar] small count {n_if}", flush=True)
+    rc, rf = (t.cpu().numpy() for t in ms.compute_mortar_flux(kind, PARAMS[kind], d(Uc), d(Uf),
+                                                               d(nrm)))
+    assert rc.shape == (n_if, nm) and rf.shape == (n_if, nfine, nm)
+    for k in range(n_if):
+        ec, ef = orc.mortar_flux(order, ctype, kind, PARAMS[kind], Uc[k], Uf[k], nrm[k])
+        s = max(np.abs(ec).max(), np.abs(ef).max())
+        np.testing.assert_allclose(rc[k], ec, rtol=0, atol=1e-12 * s)
+        np.testing.assert_allclose(rf[k], ef, rtol=0, atol=1e-12 * s)
+
+
+def test_mortar_matrix_core_grid_stride(torch, orc):
+    """more interfaces than one pass of the grid (4096 workgroups x 64)"""
+    from drifter_amd import interface as itf
+
+    order, ctype, kind = 2, 1, 5
+    ms = itf.MortarSpace(order, ctype, device=0)
+    nm, nfine = ms.num_dofs(), ms.num_fine_elements()
+    n_if = 4096 * 64 + 4099
+    g = torch.Generator(device="cuda").manual_seed(5)
+    Uc = torch.randn((n_if, nm), dtype=torch.float64, device="cuda", generator=g)
+    Uf = torch.randn((n_if, nfine, nm), dtype=torch.float64, device="cuda", generator=g)
+    nrm = torch.zeros((n_if, 3), dtype=torch.float64, device="cuda")
+    nrm[:, 0], nrm[:, 1] = 0.6, 0.8
+    print(f"[mortar] grid stride n_if={n_if}", flush=True)
+    rc, rf = ms.compute_mortar_flux(kind, PARAMS[kind], Uc, Uf, nrm)
+    torch.cuda.synchronize()
+    ks = [0, 262143, 262144, 262160, n_if - 1]
+    Ucn, Ufn, nn = Uc[ks].cpu().numpy(), Uf[ks].cpu().numpy(), nrm[ks].cpu().numpy()
+    rcn, rfn = rc[ks].cpu().numpy(), rf[ks].cpu().numpy()
+    for i, k in enumerate(ks):
+        ec, ef = orc.mortar_flux(order, ctype, kind, PARAMS[kind], Ucn[i], Ufn[i], nn[i])
+        s = max(np.abs(ec).max(), np.abs(ef).max())
+        np.testing.assert_allclose(rcn[i], ec, rtol=0, atol=1e-12 * s)
+        np.testing.assert_allclose(rfn[i], ef, rtol=0, atol=1e-12 * s)
