@@ -14,6 +14,8 @@
 // quirky column layout the column of ih is the contiguous DOF run
 // ih*n .. ih*n+n-1 (:143), so each thread streams exactly one run.  The 2D
 // operators D_x2d / D_y2d (:90-119) act on the fast / slow digit of ih.
+// The state and the predictor stage of a substep live in separate LDS
+// buffers, two barriers per substep.
 #include "drf_internal.hpp"
 
 namespace drf {
@@ -36,9 +38,16 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
 {
     using C = MsCfg<N>;
     constexpr int n = C::n, nh = C::nh, nd = C::nd, EPB = C::EPB;
+    // state (s_a, s_b, s_c) and predictor stage (t_a, t_b, t_c) of the substep
     __shared__ double s_a[EPB][nh], s_b[EPB][nh], s_c[EPB][nh];
+    __shared__ double t_a[EPB][nh], t_b[EPB][nh], t_c[EPB][nh];
+    // D staged once: read from the kernel arguments with a lane-dependent
+    // index it became 12 global loads in every substep's dependent chain
+    __shared__ double sD[n * n];
 
     const int tid = threadIdx.x;
+    for (int q = tid; q < n * n; q += blockDim.x)
+        sD[q] = ops.D[q]; // visible after the first barrier, before any use
     const bool in_tile = tid < C::NT;
     const int le = in_tile ? tid / nh : 0;
     const int ih = in_tile ? tid - le * nh : 0;
@@ -101,47 +110,46 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
     double eavg = 0.0;
     double Ufin = 0.0, Vfin = 0.0;
     for (int sub = 0; sub < nsub; ++sub) {
-        double de0, dU0, dV0;
-        // barotropic_rhs on the current state (:177-211)
+        double de0 = 0.0, dU0 = 0.0, dV0 = 0.0;
+        // barotropic_rhs on the current state (:177-211).  s_* was last read
+        // before the previous substep's second barrier.
         if (act) {
             s_a[le][ih] = HUb;
             s_b[le][ih] = HVb;
             s_c[le][ih] = eb;
         }
         __syncthreads();
+        double es = 0.0, HUs = 0.0, HVs = 0.0, Hs = 0.0;
         if (act) {
             double dx = 0.0, dy = 0.0, ex = 0.0, ey = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                dx += ops.D[a * n + m] * s_a[le][m + n * b];
-                dy += ops.D[b * n + m] * s_b[le][a + n * m];
-                ex += ops.D[a * n + m] * s_c[le][m + n * b];
-                ey += ops.D[b * n + m] * s_c[le][a + n * m];
+                dx += sD[a * n + m] * s_a[le][m + n * b];
+                dy += sD[b * n + m] * s_b[le][a + n * m];
+                ex += sD[a * n + m] * s_c[le][m + n * b];
+                ey += sD[b * n + m] * s_c[le][a + n * m];
             }
             de0 = -(dx + dy);
             dU0 = -g * Hb * ex + fh * HVb + 0.0;
             dV0 = -g * Hb * ey + -fh * HUb + 0.0;
-        }
-        __syncthreads();
-        double es = 0.0, HUs = 0.0, HVs = 0.0, Hs = 0.0;
-        if (act) {
             es = eb + dtb * de0;
             HUs = HUb + dtb * dU0;
             HVs = HVb + dtb * dV0;
             Hs = es + hh; // BarotropicState::update_derived :31
-            s_a[le][ih] = HUs;
-            s_b[le][ih] = HVs;
-            s_c[le][ih] = es;
+            // t_* was last read before this substep's first barrier
+            t_a[le][ih] = HUs;
+            t_b[le][ih] = HVs;
+            t_c[le][ih] = es;
         }
         __syncthreads();
         if (act) {
             double dx = 0.0, dy = 0.0, ex = 0.0, ey = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                dx += ops.D[a * n + m] * s_a[le][m + n * b];
-                dy += ops.D[b * n + m] * s_b[le][a + n * m];
-                ex += ops.D[a * n + m] * s_c[le][m + n * b];
-                ey += ops.D[b * n + m] * s_c[le][a + n * m];
+                dx += sD[a * n + m] * t_a[le][m + n * b];
+                dy += sD[b * n + m] * t_b[le][a + n * m];
+                ex += sD[a * n + m] * t_c[le][m + n * b];
+                ey += sD[b * n + m] * t_c[le][a + n * m];
             }
             const double de1 = -(dx + dy);
             const double dU1 = -g * Hs * ex + fh * HVs + 0.0;
@@ -150,12 +158,14 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
             HUb += 0.5 * dtb * (dU0 + dU1);
             HVb += 0.5 * dtb * (dV0 + dV1);
             Hb = eb + hh;
-            const double hi = hinv(Hb);
-            Ufin = HUb * hi;
-            Vfin = HVb * hi;
             eavg += eb;
         }
-        __syncthreads();
+    }
+    // U_bar = HU / H of the last substep (update_derived :32-34)
+    if (act) {
+        const double hi = hinv(Hb);
+        Ufin = HUb * hi;
+        Vfin = HVb * hi;
     }
     eavg /= (double)nsub;
 
@@ -164,9 +174,10 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
     double ut[n], vt[n];
     if (act) {
         Hn = eavg + hh;
+        // eta := eta_avg here (:509) is overwritten by recombine's final
+        // barotropic eta (:423) before anything reads it: only that is stored
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-            eta[c0 + k] = eavg;
             H[c0 + k] = Hn;
             ut[k] = Ub + up[k];
             vt[k] = Vb + vp[k];
@@ -197,8 +208,8 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
         double dx = 0.0, dy = 0.0;
 #pragma unroll
         for (int m = 0; m < n; ++m) {
-            dx += ops.D[a * n + m] * s_a[le][m + n * b];
-            dy += ops.D[b * n + m] * s_b[le][a + n * m];
+            dx += sD[a * n + m] * s_a[le][m + n * b];
+            dy += sD[b * n + m] * s_b[le][a + n * m];
         }
         const double div = dx + dy;
         const double hi = hinv(Hn); // H(ih * n_vert), :575
