@@ -118,6 +118,17 @@ SIGNATURES = {
     "drf_column_sigma_nodes": (_I, [_I, _PD]),
     "drf_diagnose_column_dev": (_I, [_I, _L, _P, _P, _P, _P, _P]),
     "drf_continuity_residual_dev": (_I, [_I, _L, _P, _P, _L, _P, _P]),
+    "drf_vtk_create": (_I, [C.c_char_p, _I, _I, C.POINTER(_P)]),
+    "drf_vtk_destroy": (None, [_P]),
+    "drf_vtk_set_polynomial_order": (_I, [_P, _I]),
+    "drf_vtk_set_mesh": (_I, [_P, _L, _PD]),
+    "drf_vtk_add_field": (_I, [_P, C.c_char_p, _I, _I]),
+    "drf_vtk_set_field": (_I, [_P, C.c_char_p, _I, _L, _PD]),
+    "drf_vtk_set_point_field_dev": (_I, [_P, C.c_char_p, _L, _P, _L, _L, _P]),
+    "drf_vtk_filename": (_I, [_P, _L, C.c_char_p, _L]),
+    "drf_vtk_write": (_I, [_P, _L, _D]),
+    "drf_vtk_write_timestep": (_I, [_P, _D]),
+    "drf_vtk_finalize": (_I, [_P]),
     "drf_rhs_bytes": (_D, [_P]),
     "drf_num_elements": (_L, [_P]),
     "drf_order": (_I, [_P]),

@@ -558,6 +558,48 @@ drf_status drf_continuity_residual_dev(int order, int64_t n_elem, const void *di
                                        void *residual, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* VTK output (VTKWriter, include/io/vtk_writer.hpp,                    */
+/* src/io/vtk_writer.cpp:47-475): host-side writer of .vtu / .pvd files */
+/* byte-identical to the reference's, fields taken from device memory   */
+/* ------------------------------------------------------------------ */
+
+typedef struct drf_vtk drf_vtk;
+#define DRF_VTK_VTU 0    /* VTKFormat::VTU */
+#define DRF_VTK_PVTU 1   /* VTKFormat::PVTU: one piece without MPI (:367-418) */
+#define DRF_VTK_ASCII 0  /* VTKEncoding */
+#define DRF_VTK_BINARY 1 /* written as base64, as the reference does (:454-458) */
+#define DRF_VTK_BASE64 2 /* base64; the header names a zlib compressor it does not use */
+
+/* VTKWriter(basename, format, encoding) (:47-63): creates basename's directory */
+drf_status drf_vtk_create(const char *basename, int format, int encoding, drf_vtk **out);
+void drf_vtk_destroy(drf_vtk *w);
+/* set_polynomial_order (:65); takes effect at the next drf_vtk_set_mesh */
+drf_status drf_vtk_set_polynomial_order(drf_vtk *w, int order);
+/* set_mesh + build_mesh_geometry (:67-143) from leaf bounds (E*6, as
+ * drf_mesh_elements writes them): equispaced nodes per element (at order 1
+ * the reference coordinate is the node index, 0 or 1), linear hexahedra in
+ * VTK corner order at order 1, Lagrange hexahedra in DG node order above */
+drf_status drf_vtk_set_mesh(drf_vtk *w, int64_t n_elem, const double *bounds);
+/* add_point_data / add_cell_data (:145-159) (on_cells 0 / 1) */
+drf_status drf_vtk_add_field(drf_vtk *w, const char *name, int ncomp, int on_cells);
+/* set_point_data / set_cell_data (:161-212) from host memory; DRF_ERR_ARG
+ * for an unknown name ("Unknown point field: name", as the reference throws) */
+drf_status drf_vtk_set_field(drf_vtk *w, const char *name, int on_cells, int64_t n,
+                             const double *data);
+/* set_point_data(name, vector<VecX>) (:173-192) from a device field: ndof
+ * values per element at field + e*elem_stride (one strided copy) */
+drf_status drf_vtk_set_point_field_dev(drf_vtk *w, const char *name, int64_t n_elem,
+                                       const void *field, int64_t elem_stride, int64_t ndof,
+                                       void *stream);
+/* get_filename (:231-249): basename_NNNNNN.vtu */
+drf_status drf_vtk_filename(const drf_vtk *w, int64_t time_idx, char *buf, int64_t buf_len);
+/* write (:214-227) and write_timestep (:229) */
+drf_status drf_vtk_write(drf_vtk *w, int64_t time_idx, double time);
+drf_status drf_vtk_write_timestep(drf_vtk *w, double time);
+/* finalize: the .pvd collection of the written steps (:420-437) */
+drf_status drf_vtk_finalize(drf_vtk *w);
+
+/* ------------------------------------------------------------------ */
 /* Introspection for measurement                                       */
 /* ------------------------------------------------------------------ */
 /* Algorithmic HBM bytes of one drf_rhs_dev launch at the current mesh. */
