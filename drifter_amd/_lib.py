@@ -82,6 +82,8 @@ SIGNATURES = {
     "drf_mortar_num_dofs": (_I, [_P]),
     "drf_mortar_operators": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "drf_mortar_flux_batch": (_I, [_P, _I, _PD, _L, _P, _P, _P, _P, _P, _P]),
+    "drf_verify_mortar_conservation": (_I, [_P, _PD, _PD, _PD, _PD, _PD]),
+    "drf_mortar_conservation_dev": (_I, [_P, _L, _P, _P, _P, _P, _P, _P]),
     "drf_estimator_create": (_I, [_I, _D, _D, _D, _I, _I, _I, C.POINTER(_P)]),
     "drf_estimator_destroy": (None, [_P]),
     "drf_element_char_length": (_I, [_L, _PD, _PD]),

@@ -499,6 +499,26 @@ hipError_t launch_mortar_mfma(int kind, const FluxParams &p, int nfine, const do
     return hipGetLastError();
 }
 
+// verify_mortar_conservation (src/dg/mortar.cpp:386-413), one interface per
+// lane: the weighted coarse and fine integrals in the reference's order.
+__global__ void __launch_bounds__(256)
+    mortar_conservation_kernel(int nm, int nfine, int64_t nif, const double *__restrict__ rc,
+                               const double *__restrict__ rf, const double *__restrict__ wc,
+                               const double *__restrict__ wf, double *__restrict__ out)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nif;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double ic = 0.0, ifn = 0.0;
+        for (int i = 0; i < nm; ++i)
+            ic += wc[i] * rc[k * nm + i];
+        for (int f = 0; f < nfine; ++f)
+            for (int i = 0; i < nm; ++i)
+                ifn += wf[f * nm + i] * rf[(k * nfine + f) * nm + i];
+        const double total = ic + ifn, norm = fabs(ic) + fabs(ifn);
+        out[k] = (norm < 1e-15) ? 0.0 : fabs(total) / norm;
+    }
+}
+
 // Face traces of a nodal field: out[k][q] = field[elems[k] * es + node(face[k], q)].
 __global__ void __launch_bounds__(256)
     gather_faces_kernel(int n, int64_t nelem, int64_t es, int64_t nfaces,
@@ -567,6 +587,18 @@ hipError_t launch_mortar(int kind, const FluxParams &p, int nm, int nfine, const
     else
         hipLaunchKernelGGL(mortar_kernel<false>, dim3(grid), dim3(256), base, s, kind, p, nm,
                            nfine, ops, nif, Uc, Uf, nrm, rc, rf);
+    return hipGetLastError();
+}
+
+hipError_t launch_mortar_conservation(int nm, int nfine, int64_t nif, const double *rc,
+                                     const double *rf, const double *wc, const double *wf,
+                                     double *out, hipStream_t s)
+{
+    if (nif <= 0)
+        return hipSuccess;
+    const int64_t b = (nif + 255) / 256;
+    hipLaunchKernelGGL(mortar_conservation_kernel, dim3((unsigned)(b < 65536 ? b : 65536)),
+                       dim3(256), 0, s, nm, nfine, nif, rc, rf, wc, wf, out);
     return hipGetLastError();
 }
 

@@ -82,6 +82,9 @@ hipError_t launch_flux_points(int kind, int64_t npts, const double *UL, const do
 hipError_t launch_conforming(int kind, const FluxParams &p, int nf, const FaceRule &rule,
                              int64_t nfaces, const double *UL, const double *UR,
                              const double *nrm, double *rl, double *rr, hipStream_t s);
+hipError_t launch_mortar_conservation(int nm, int nfine, int64_t nif, const double *rc,
+                                     const double *rf, const double *wc, const double *wf,
+                                     double *out, hipStream_t s);
 hipError_t launch_mortar(int kind, const FluxParams &p, int nm, int nfine, const double *ops,
                          int64_t nif, const double *Uc, const double *Uf, const double *nrm,
                          double *rc, double *rf, hipStream_t s);

@@ -405,4 +405,38 @@ drf_status drf_mortar_flux_batch(drf_mortar *m, int kind, const double *params,
     return DRF_OK;
 }
 
+drf_status drf_verify_mortar_conservation(const drf_mortar *m, const double *rhs_coarse,
+                                          const double *rhs_fine, const double *w_coarse,
+                                          const double *w_fine, double *rel_error)
+{
+    DRF_REQUIRE(m && rhs_coarse && rhs_fine && w_coarse && w_fine && rel_error, DRF_ERR_ARG,
+                "drf_verify_mortar_conservation: null argument");
+    double ic = 0.0, ifn = 0.0;
+    for (int i = 0; i < m->nm; ++i)
+        ic += w_coarse[i] * rhs_coarse[i];
+    for (int f = 0; f < m->nfine; ++f)
+        for (int i = 0; i < m->nm; ++i)
+            ifn += w_fine[f * m->nm + i] * rhs_fine[f * m->nm + i];
+    const double total = ic + ifn, norm = std::fabs(ic) + std::fabs(ifn);
+    *rel_error = (norm < 1e-15) ? 0.0 : std::fabs(total) / norm;
+    return DRF_OK;
+}
+
+drf_status drf_mortar_conservation_dev(const drf_mortar *m, int64_t n_interfaces,
+                                       const void *rhs_c, const void *rhs_f, const void *w_coarse,
+                                       const void *w_fine, void *rel_error, void *stream)
+{
+    DRF_REQUIRE(m, DRF_ERR_ARG, "drf_mortar_conservation_dev: null mortar");
+    DRF_REQUIRE(n_interfaces >= 0, DRF_ERR_ARG, "drf_mortar_conservation_dev: negative count");
+    if (n_interfaces == 0)
+        return DRF_OK;
+    DRF_REQUIRE(rhs_c && rhs_f && w_coarse && w_fine && rel_error, DRF_ERR_ARG,
+                "drf_mortar_conservation_dev: null pointer");
+    DRF_HIP_CHECK(launch_mortar_conservation(m->nm, m->nfine, n_interfaces, (const double *)rhs_c,
+                                             (const double *)rhs_f, (const double *)w_coarse,
+                                             (const double *)w_fine, (double *)rel_error,
+                                             (hipStream_t)stream));
+    return DRF_OK;
+}
+
 } // extern "C"

@@ -153,3 +153,34 @@ class MortarSpace:
             _dev(Uf, n * nfine * nm, "Uf"), _dev(normals, 3 * n, "normals"), _ptr(rc),
             _ptr(rf), _sptr(stream)))
         return rc, rf
+
+    def verify_conservation(self, rhs_coarse, rhs_fine, w_coarse, w_fine):
+        """verify_mortar_conservation (src/dg/mortar.cpp:386-413) for one
+        interface (host numpy arrays): relative imbalance of the weighted
+        coarse and fine integrals, 0 when both are below 1e-15 together."""
+        nm, nfine = self.num_dofs(), self.num_fine_elements()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                for a in (rhs_coarse, rhs_fine, w_coarse, w_fine)]
+        for a, n, name in zip(arrs, (nm, nfine * nm, nm, nfine * nm),
+                              ("rhs_coarse", "rhs_fine", "w_coarse", "w_fine")):
+            if a.size != n:
+                raise ValueError(f"{name}: expected {n} values, got {a.size}")
+        out = C.c_double()
+        check(self._l.drf_verify_mortar_conservation(
+            self._m, *(a.ctypes.data_as(_lib._PD) for a in arrs), C.byref(out)))
+        return out.value
+
+    def verify_conservation_batch(self, rhs_coarse, rhs_fine, w_coarse, w_fine, stream=None):
+        """The same for a batch on the device: rhs_coarse [n][nm], rhs_fine
+        [n][nfine][nm] (as compute_mortar_flux returns them), weights shared by
+        every interface -> relative errors [n]."""
+        import torch
+
+        nm, nfine = self.num_dofs(), self.num_fine_elements()
+        n = rhs_coarse.shape[0]
+        out = torch.empty(n, dtype=torch.float64, device=rhs_coarse.device)
+        check(self._l.drf_mortar_conservation_dev(
+            self._m, n, _dev(rhs_coarse, n * nm, "rhs_coarse"),
+            _dev(rhs_fine, n * nfine * nm, "rhs_fine"), _dev(w_coarse, nm, "w_coarse"),
+            _dev(w_fine, nfine * nm, "w_fine"), _ptr(out), _sptr(stream)))
+        return out

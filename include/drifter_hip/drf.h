@@ -192,7 +192,9 @@ drf_status drf_graph_rk_adaptive(drf_ctx *ctx, int scheme, double cfl, double dx
  * n_steps x n_sub substeps. */
 drf_status drf_graph_mode_split(drf_ctx *ctx, double dt_3d, int n_sub, void *full, int n_steps,
                                 drf_graph **out);
-/* Replays the captured steps on `stream` (asynchronous). */
+/* Replays the captured steps on `stream` (asynchronous).  Every graph keeps
+ * the device addresses of its context's buffers (coefficients, reduction
+ * buffer): destroy it before drf_initialize or drf_destroy on that context. */
 drf_status drf_graph_launch(drf_graph *g, void *stream);
 void drf_graph_destroy(drf_graph *g);
 
@@ -305,6 +307,18 @@ drf_status drf_mortar_operators(const drf_mortar *m, double *M, double *Minv, do
 drf_status drf_mortar_flux_batch(drf_mortar *m, int kind, const double *params,
                                  int64_t n_interfaces, const void *Uc, const void *Uf,
                                  const void *normals, void *rhs_c, void *rhs_f, void *stream);
+/* verify_mortar_conservation (src/dg/mortar.cpp:386-413) for one interface,
+ * host: |sum_i w_coarse_i rhs_coarse_i + sum_f sum_i w_fine_fi rhs_fine_fi| /
+ * (|coarse integral| + |fine integral|), 0 when that norm is below 1e-15.
+ * rhs_fine and w_fine are [nfine][(N+1)^2]. */
+drf_status drf_verify_mortar_conservation(const drf_mortar *m, const double *rhs_coarse,
+                                          const double *rhs_fine, const double *w_coarse,
+                                          const double *w_fine, double *rel_error);
+/* The same for a batch (device; rhs_c / rhs_f as drf_mortar_flux_batch writes
+ * them, the weights shared by every interface): rel_error[n_interfaces] */
+drf_status drf_mortar_conservation_dev(const drf_mortar *m, int64_t n_interfaces,
+                                       const void *rhs_c, const void *rhs_f, const void *w_coarse,
+                                       const void *w_fine, void *rel_error, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Error estimator and element marking (GradientErrorEstimator,        */

@@ -536,3 +536,21 @@ int orc_mortar_flux(int order, int type, int kind, const double *p, const double
     free(M);
     return nfine;
 }
+
+/* verify_mortar_conservation, src/dg/mortar.cpp:386-413: |sum_i wc_i rc_i +
+ * sum_f sum_i wf_fi rf_fi| / (|coarse integral| + |fine integral|), 0 when
+ * that norm is below 1e-15 */
+double orc_verify_mortar_conservation(int nm, int nfine, const double *rc, const double *rf,
+                                      const double *wc, const double *wf)
+{
+    double ic = 0.0, ifn = 0.0;
+    for (int i = 0; i < nm; ++i)
+        ic += wc[i] * rc[i];
+    for (int f = 0; f < nfine; ++f)
+        for (int i = 0; i < nm; ++i)
+            ifn += wf[f * nm + i] * rf[f * nm + i];
+    const double total = ic + ifn, norm = fabs(ic) + fabs(ifn);
+    if (norm < 1e-15)
+        return 0.0;
+    return fabs(total) / norm;
+}

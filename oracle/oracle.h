@@ -172,6 +172,9 @@ double orc_bf16_round(double x);
 double orc_mlp_loss_grad(const double *params, int64_t B, const double *X, const double *y,
                          int quantize, double *pred, double *grad);
 
+double orc_verify_mortar_conservation(int nm, int nfine, const double *rc, const double *rf,
+                                      const double *wc, const double *wf);
+
 /* ---- AMR solution transfer (SolutionProjection, transfer_solution) ---- */
 /* children: nchild x ndof, child index cx + nx (cy + ny cz); returns nchild */
 int orc_project_to_children(int order, int mask, const double *U, double *children);

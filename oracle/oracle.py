@@ -74,6 +74,7 @@ def load():
         "orc_modal_coeffs": (C.c_int, [C.c_int, C.c_int64, _PD, C.c_int64, _PD]),
         "orc_sigma_metrics": (None, [C.c_int64] + [_PD] * 13),
         "orc_bf16_round": (C.c_double, [C.c_double]),
+        "orc_verify_mortar_conservation": (C.c_double, [C.c_int, C.c_int] + [_PD] * 4),
         "orc_project_to_children": (C.c_int, [C.c_int, C.c_int, _PD, _PD]),
         "orc_diagnose_omega": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, C.c_int64, _PD]),
         "orc_diagnose_omega_from_velocity": (C.c_int, [C.c_int, C.c_int64, _PD, _PD, _PD,
@@ -472,3 +473,10 @@ def sigma_ops_apply(order, op, U, a=None, b=None, c=None, f=None, g=None, dsdx=N
     load().orc_sigma_ops_apply(order, op, E, _vp(U), *[_vp(x) for x in conv], nd, _vp(o1),
                                _vp(o2))
     return (o1, o2) if op == SIGMA_OP_HGRAD else o1
+
+
+def verify_mortar_conservation(rc, rf, wc, wf):
+    """rc (nm,), rf (nfine, nm), wc (nm,), wf (nfine, nm) -> relative error"""
+    rc, rf, wc, wf = (_v(a) for a in (rc, rf, wc, wf))
+    return load().orc_verify_mortar_conservation(rc.size, rf.shape[0], _p(rc), _p(rf), _p(wc),
+                                                 _p(wf))

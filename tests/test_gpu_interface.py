@@ -214,3 +214,32 @@ def test_mortar_matrix_core_grid_stride(torch, orc):
         s = max(np.abs(ec).max(), np.abs(ef).max())
         np.testing.assert_allclose(rcn[i], ec, rtol=0, atol=1e-12 * s)
         np.testing.assert_allclose(rfn[i], ef, rtol=0, atol=1e-12 * s)
+
+
+@pytest.mark.parametrize("ctype", [0, 2, 4, 5])
+def test_mortar_conservation_batch_bit_exact(torch, orc, ctype):
+    """batched verify_mortar_conservation on the mortar flux's own output,
+    bit-exact against the oracle per interface; empty batch is a no-op"""
+    from drifter_amd import interface as itf
+
+    order, kind = 3, 6
+    print(f"[mortar conservation] type={ctype}", flush=True)
+    ms = itf.MortarSpace(order, ctype, device=0)
+    nm, nfine = ms.num_dofs(), ms.num_fine_elements()
+    n_if = 300
+    rng = np.random.default_rng(40 + ctype)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rc, rf = ms.compute_mortar_flux(kind, PARAMS[kind], d(rng.normal(size=(n_if, nm))),
+                                    d(rng.normal(size=(n_if, nfine, nm))),
+                                    d(_unit_normals(rng, n_if, planar=False)))
+    rc[0].zero_()
+    rf[0].zero_()
+    wc, wf = rng.uniform(0, 1, nm), rng.uniform(0, 1, (nfine, nm))
+    got = ms.verify_conservation_batch(rc, rf, d(wc), d(wf)).cpu().numpy()
+    rcn, rfn = rc.cpu().numpy(), rf.cpu().numpy()
+    exp = np.array([orc.verify_mortar_conservation(rcn[k], rfn[k], wc, wf)
+                    for k in range(n_if)])
+    np.testing.assert_array_equal(got, exp)
+    assert got[0] == 0.0
+    empty = ms.verify_conservation_batch(rc[:0], rf[:0], d(wc), d(wf))
+    assert empty.numel() == 0

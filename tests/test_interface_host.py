@@ -70,3 +70,19 @@ def test_flux_entry_points_reject_bad_arguments():
     assert lib.drf_mortar_flux_batch(m, 5, pd, 1, vp, vp, vp, vp, vp,
                                      None) == _lib.DRF_ERR_STATE  # host-only mortar
     lib.drf_mortar_destroy(m)
+
+
+@pytest.mark.parametrize("ctype", [0, 1, 3, 5])
+def test_verify_conservation_host_bit_exact(orc, ctype):
+    """drf_verify_mortar_conservation against the oracle, same summation order"""
+    ms = itf.MortarSpace(3, ctype, device=None)
+    nm, nfine = ms.num_dofs(), ms.num_fine_elements()
+    rng = np.random.default_rng(ctype)
+    for _ in range(5):
+        rc, rf = rng.normal(size=nm), rng.normal(size=(nfine, nm))
+        wc, wf = rng.uniform(0, 1, nm), rng.uniform(0, 1, (nfine, nm))
+        assert ms.verify_conservation(rc, rf, wc, wf) == orc.verify_mortar_conservation(
+            rc, rf, wc, wf)
+    assert ms.verify_conservation(np.zeros(nm), np.zeros((nfine, nm)), wc, wf) == 0.0
+    with pytest.raises(ValueError):
+        ms.verify_conservation(rc[:-1], rf, wc, wf)

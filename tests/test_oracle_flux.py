@@ -217,3 +217,61 @@ def test_mortar_flux_same_level_is_antisymmetric(orc, order):
     # lifted flux integrates to the sum of nodal fluxes (partition of unity)
     F = 0.5 * 0.1 * (Uc + Uf[0]) - 0.5 * 0.8 * (Uf[0] - Uc)
     assert abs(rc.sum() - F.sum()) < 1e-12 * (1 + np.abs(F).sum())
+
+
+# verify_mortar_conservation (src/dg/mortar.cpp:386-413).  No reference test
+# calls it; these closed forms pin the oracle: exact dyadic cancellation,
+# per-fine-element weights (a dropped fine index or the coarse weights used
+# on the fine side change the ratio), the 1e-15 floor and scale invariance.
+def test_conservation_exact_cancellation(orc):
+    nm, nfine = 9, 4
+    rc = np.arange(1, nm + 1, dtype=float)
+    wc = np.full(nm, 0.5)
+    rf = np.tile(-rc / 8.0, (nfine, 1))
+    wf = np.ones((nfine, nm))
+    assert orc.verify_mortar_conservation(rc, rf, wc, wf) == 0.0
+
+
+def test_conservation_closed_form_ratio(orc):
+    nm, nfine = 16, 4
+    rc, wc = np.ones(nm), np.ones(nm)
+    rf = -np.ones((nfine, nm))
+    wf = np.array([(f + 1) / 8.0 for f in range(nfine)])[:, None] * np.ones((nfine, nm))
+    # coarse nm, fine -nm (1+2+3+4)/8 = -1.25 nm: |-0.25 nm| / (2.25 nm) = 1/9
+    assert abs(orc.verify_mortar_conservation(rc, rf, wc, wf) - 1.0 / 9.0) < 1e-16
+    # same signs on both sides: the ratio is one
+    assert orc.verify_mortar_conservation(rc, -rf, wc, wf) == 1.0
+
+
+def test_conservation_norm_floor(orc):
+    nm, nfine = 4, 2
+    tiny = np.full(nm, 1e-17)
+    assert orc.verify_mortar_conservation(tiny, np.zeros((nfine, nm)), np.ones(nm),
+                                          np.ones((nfine, nm))) == 0.0
+    assert orc.verify_mortar_conservation(np.zeros(nm), np.zeros((nfine, nm)), np.ones(nm),
+                                          np.ones((nfine, nm))) == 0.0
+    # just above the floor the ratio is reported (one-sided flux: 1)
+    above = np.full(nm, 1e-15)
+    assert orc.verify_mortar_conservation(above, np.zeros((nfine, nm)), np.ones(nm),
+                                          np.ones((nfine, nm))) == 1.0
+
+
+def test_conservation_scale_invariant(orc):
+    rng = np.random.default_rng(5)
+    nm, nfine = 9, 3
+    rc, rf = rng.normal(size=nm), rng.normal(size=(nfine, nm))
+    wc, wf = rng.uniform(0.1, 1, nm), rng.uniform(0.1, 1, (nfine, nm))
+    r = orc.verify_mortar_conservation(rc, rf, wc, wf)
+    assert 0.0 <= r <= 1.0
+    assert orc.verify_mortar_conservation(rc * 1024.0, rf * 1024.0, wc, wf) == r
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_conservation_same_level_mortar_flux(orc, order):
+    """a same-level mortar lifts +F to one side and -F to the other, so the
+    unit-weight integrals cancel to rounding"""
+    nm = (order + 1) ** 2
+    rng = np.random.default_rng(order)
+    rc, rf = orc.mortar_flux(order, 0, 6, [0.5, 0.1, -0.3, 0.8], rng.normal(size=nm),
+                             rng.normal(size=(1, nm)), [0.0, 1.0, 0.0])
+    assert orc.verify_mortar_conservation(rc, rf, np.ones(nm), np.ones((1, nm))) < 1e-14
