@@ -256,7 +256,8 @@ def measure_mode_split(case, stream):
 
 def measure_faces(stream):
     """Face components at coastline-mesh scale, N = 3: pointwise fluxes,
-    conforming faces, 2:1 mortars (2x2 sub-faces) and face-trace gathers."""
+    conforming faces, 2:1 mortars (2x2 sub-faces; also at N = 4) and
+    face-trace gathers."""
     import numpy as np
     import torch
 
@@ -292,6 +293,12 @@ def measure_faces(stream):
     ms = time_launches(lambda: m.compute_mortar_flux(itf.LF_ADV, [0.3, 0.2, 0.1, 0.5], Uc, Uf, mn,
                                                      stream), 20, 3, stream)
     out["mortar_2x2_100k_interfaces_n3_us"] = round(ms * 1e3, 1)
+    m4 = itf.MortarSpace(4, itf.CONN_FINE_2X2)
+    Uc4 = torch.rand((nif, 25), device=dev, dtype=torch.float64, generator=g)
+    Uf4 = torch.rand((nif, 4, 25), device=dev, dtype=torch.float64, generator=g)
+    ms = time_launches(lambda: m4.compute_mortar_flux(itf.LF_ADV, [0.3, 0.2, 0.1, 0.5], Uc4, Uf4,
+                                                      mn, stream), 20, 3, stream)
+    out["mortar_2x2_100k_interfaces_n4_us"] = round(ms * 1e3, 1)
     E = 39280
     field = torch.rand((E, 64), device=dev, dtype=torch.float64, generator=g)
     el = torch.arange(E, device=dev, dtype=torch.int64).repeat_interleave(6)

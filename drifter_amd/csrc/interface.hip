@@ -10,9 +10,9 @@
 //   MortarSpace::compute_mortar_flux    src/dg/mortar.cpp:212-258
 //   LGL face extraction                 src/dg/basis_hexahedron.cpp:431-522
 // Pointwise fluxes and conforming faces run one face point per lane.  Mortars
-// of orders 1-3 run on the FP64 matrix cores: 16 interfaces per wavefront,
-// each operator product a 16 x 16 x (N+1)^2 GEMM with the operators in LDS.
-// Higher orders run persistent workgroups over the interfaces of one
+// of orders 1-4 run on the FP64 matrix cores: 16 interfaces per wavefront,
+// each operator product a (16 MT) x 16 x (N+1)^2 GEMM with the operators in
+// LDS.  Higher orders run persistent workgroups over the interfaces of one
 // connection type: the type's projection and lift operators (built once on
 // the host) are staged in LDS when they fit, and every lane owns one mortar
 // DOF of one of the workgroup's interfaces.
@@ -338,40 +338,50 @@ __device__ __forceinline__ void wave_sync_lds()
 }
 
 // MortarSpace::compute_mortar_flux on the FP64 matrix cores, for nm =
-// (N+1)^2 <= 16 (orders 1-3).  A wavefront takes 16 interfaces; every
-// operator product is a 16 x 16 x nm GEMM, operator (rows = mortar DOFs,
-// zero-padded to 16 x KP in LDS) times the 16 interfaces' vectors, in
-// v_mfma_f64_16x16x4f64 steps of K = 4.  Lane l = col + 16 kq holds
-//   A: row m = col, K index 4c + kq of step c;   B: the same K index of
-//   interface col;   D: rows kq + 4 i (i < 4) of interface col
+// (N+1)^2 <= 25 (orders 1-4).  A wavefront takes 16 interfaces; every
+// operator product is a (16 MT) x 16 x nm GEMM, operator (rows = mortar DOFs,
+// zero-padded to MT tiles of 16 rows by KP columns in LDS) times the 16
+// interfaces' vectors, in v_mfma_f64_16x16x4f64 steps of K = 4, one chain per
+// 16-row tile.  Lane l = col + 16 kq holds
+//   A: tile row m = col, K index 4c + kq of step c;   B: the same K index of
+//   interface col;   D: tile rows kq + 4 i (i < 4) of interface col
 // (the f64 variant interleaves its output rows; measured on gfx950).  Two
 // permutations make every lane's memory accesses contiguous: step c, lane
 // group kq contracts DOF KC kq + c (so a lane's KC input DOFs are
-// consecutive), and A row m is operator row 4 (m % 4) + m / 4 (so a lane's
-// four outputs are rows 4 kq .. 4 kq + 3).  The fine projections accumulate
-// into one chain (K = nfine nm); the flux is taken on the D layout, and M^-1 F
-// and the lifts read their B operand back from a per-wave 16 x 16 LDS tile.
-// Products round as the matrix cores round them (within 1e-13 of the
-// sequential sums).
-constexpr int kMortarWaves = 4; // wavefronts per workgroup, sharing one operator copy
+// consecutive), and A row m of tile t is operator row 16 t + 4 (m % 4) + m / 4
+// (so a lane's four outputs of tile t are rows 16 t + 4 kq .. 16 t + 4 kq + 3).
+// The fine projections accumulate into one chain per tile (K = nfine nm); the
+// flux is taken on the D layout, and M^-1 F and the lifts read their B operand
+// back from a per-wave (16 MT) x 16 LDS tile.  Products round as the matrix
+// cores round them (within 1e-13 of the sequential sums).
+//
+// Orders 1-3 (one tile) run four wavefronts per workgroup; order 4 (two
+// tiles, up to 82 KB of operators for 2 x 2 sub-faces, 144 KB of LDS with
+// the flux tiles) runs sixteen, so that one workgroup's operator copy serves
+// four wavefronts per SIMD (126 VGPRs fit the 128 that allows; measured
+// 62.8 us for 100,000 2 x 2 interfaces, against 76.3 us with eight).
+template <int NM> constexpr int mortar_waves() { return NM > 16 ? 16 : 4; }
 
 template <int NM, int NF>
-__global__ void __launch_bounds__(64 * kMortarWaves)
+__global__ void __launch_bounds__(64 * mortar_waves<NM>())
     mortar_mfma_kernel(int kind, FluxParams p, const double *__restrict__ ops, int64_t nif,
                        const double *__restrict__ Uc, const double *__restrict__ Uf,
                        const double *__restrict__ nrm, double *__restrict__ rc,
                        double *__restrict__ rf)
 {
+    constexpr int W = mortar_waves<NM>(), MT = (NM + 15) / 16, NR = 16 * MT;
     constexpr int KC = (NM + 3) / 4, KP = 4 * KC, NMAT = 3 + 2 * NF, MM = NM * NM;
-    // one padded, permuted operator: 16 rows of KP, row stride KP + 1 (odd,
+    static_assert(KP <= NR, "the K index of M^-1 F must stay inside the flux tile");
+    // one padded, permuted operator: NR rows of KP, row stride KP + 1 (odd,
     // so the 16 rows an A fragment reads fall in different LDS banks)
-    constexpr int RS = KP + 1, OPS = 16 * RS;
+    constexpr int RS = KP + 1, OPS = NR * RS;
     __shared__ double sOp[NMAT * OPS];
-    __shared__ double sT[kMortarWaves][16 * 16];
-    // sOp[mat][m][4c + kq] = Op[4 (m % 4) + m / 4][KC kq + c]
-    for (int q = threadIdx.x; q < NMAT * OPS; q += 64 * kMortarWaves) {
+    __shared__ double sT[W][NR * 16];
+    // sOp[mat][16 t + m][4c + kq] = Op[16 t + 4 (m % 4) + m / 4][KC kq + c]
+    for (int q = threadIdx.x; q < NMAT * OPS; q += 64 * W) {
         const int m = q / OPS, r = (q - m * OPS) / RS, j = q - m * OPS - r * RS;
-        const int row = 4 * (r & 3) + (r >> 2), k = KC * (j & 3) + (j >> 2);
+        const int rr = r & 15;
+        const int row = (r & ~15) + 4 * (rr & 3) + (rr >> 2), k = KC * (j & 3) + (j >> 2);
         sOp[q] = (j < KP && row < NM && k < NM) ? ops[m * MM + row * NM + k] : 0.0;
     }
     __syncthreads();
@@ -380,8 +390,10 @@ __global__ void __launch_bounds__(64 * kMortarWaves)
     const double *Minv = sOp, *Pc = sOp + OPS, *Lc = sOp + 2 * OPS, *Pf = sOp + 3 * OPS,
                  *Lf = sOp + (3 + NF) * OPS;
     const int k0 = KC * kq; // this lane's first K index
-    for (int64_t i0 = ((int64_t)blockIdx.x * kMortarWaves + w) * 16; i0 < nif;
-         i0 += (int64_t)gridDim.x * kMortarWaves * 16) {
+    // A fragment of tile t, step c, of the operator at o
+#define MORTAR_A(o, t, c) (o)[((t) * 16 + col) * RS + 4 * (c) + kq]
+    for (int64_t i0 = ((int64_t)blockIdx.x * W + w) * 16; i0 < nif;
+         i0 += (int64_t)gridDim.x * W * 16) {
         const int64_t it = i0 + col;
         const bool valid = it < nif;
         // the lane's KC consecutive DOFs of every input face
@@ -404,70 +416,114 @@ __global__ void __launch_bounds__(64 * kMortarWaves)
                         bf[f][c] = uf[f * NM + c];
                 }
         }
-        dbl4 ac = {0.0, 0.0, 0.0, 0.0}, af = {0.0, 0.0, 0.0, 0.0};
+        dbl4 ac[MT], af[MT];
 #pragma unroll
-        for (int c = 0; c < KC; ++c)
-            ac = __builtin_amdgcn_mfma_f64_16x16x4f64(Pc[col * RS + 4 * c + kq], bc[c], ac, 0, 0,
-                                                      0);
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
+        for (int t = 0; t < MT; ++t) {
+            ac[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+            af[t] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int c = 0; c < KC; ++c)
-                af = __builtin_amdgcn_mfma_f64_16x16x4f64(Pf[f * OPS + col * RS + 4 * c + kq],
-                                                          bf[f][c], af, 0, 0, 0);
+                ac[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(MORTAR_A(Pc, t, c), bc[c], ac[t], 0,
+                                                             0, 0);
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+#pragma unroll
+                for (int c = 0; c < KC; ++c)
+                    af[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(MORTAR_A(Pf + f * OPS, t, c),
+                                                                 bf[f][c], af[t], 0, 0, 0);
+        }
         double n3[3] = {0.0, 0.0, 0.0};
         if (valid) {
             n3[0] = nrm[3 * it];
             n3[1] = nrm[3 * it + 1];
             n3[2] = nrm[3 * it + 2];
         }
-        // D register i of this lane is mortar DOF 4 kq + i
+        // D register i of tile t is mortar DOF 16 t + 4 kq + i
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * kq + i;
-            T[row * 16 + col] =
-                (valid && row < NM) ? scalar_flux(kind, ac[i], af[i], n3, p) : 0.0;
-        }
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * t + 4 * kq + i;
+                T[row * 16 + col] =
+                    (valid && row < NM) ? scalar_flux(kind, ac[t][i], af[t][i], n3, p) : 0.0;
+            }
         wave_sync_lds();
-        dbl4 am = {0.0, 0.0, 0.0, 0.0};
+        dbl4 am[MT];
 #pragma unroll
-        for (int c = 0; c < KC; ++c)
-            am = __builtin_amdgcn_mfma_f64_16x16x4f64(Minv[col * RS + 4 * c + kq],
-                                                      T[(k0 + c) * 16 + col], am, 0, 0, 0);
+        for (int t = 0; t < MT; ++t) {
+            am[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+                am[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(MORTAR_A(Minv, t, c),
+                                                             T[(k0 + c) * 16 + col], am[t], 0, 0,
+                                                             0);
+        }
         wave_sync_lds(); // every lane has read F before M^-1 F overwrites it
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            T[(4 * kq + i) * 16 + col] = am[i];
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                T[(16 * t + 4 * kq + i) * 16 + col] = am[t][i];
         wave_sync_lds();
         double bm[KC];
 #pragma unroll
         for (int c = 0; c < KC; ++c)
             bm[c] = T[(k0 + c) * 16 + col];
-        dbl4 ar = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < KC; ++c)
-            ar = __builtin_amdgcn_mfma_f64_16x16x4f64(Lc[col * RS + 4 * c + kq], bm[c], ar, 0, 0,
-                                                      0);
-        if (valid)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (4 * kq + i < NM)
-                    rc[it * NM + 4 * kq + i] = ar[i];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            dbl4 a2 = {0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < MT; ++t) {
+            dbl4 ar = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int c = 0; c < KC; ++c)
-                a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lf[f * OPS + col * RS + 4 * c + kq],
-                                                          bm[c], a2, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f64_16x16x4f64(MORTAR_A(Lc, t, c), bm[c], ar, 0, 0, 0);
             if (valid)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (4 * kq + i < NM)
-                        rf[(it * NF + f) * NM + 4 * kq + i] = -a2[i];
+                    if (16 * t + 4 * kq + i < NM)
+                        rc[it * NM + 16 * t + 4 * kq + i] = ar[i];
         }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                dbl4 a2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int c = 0; c < KC; ++c)
+                    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(MORTAR_A(Lf + f * OPS, t, c), bm[c],
+                                                              a2, 0, 0, 0);
+                if (valid)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (16 * t + 4 * kq + i < NM)
+                            rf[(it * NF + f) * NM + 16 * t + 4 * kq + i] = -a2[i];
+            }
         wave_sync_lds(); // the tile is rewritten by the next interfaces
     }
+#undef MORTAR_A
+}
+
+template <int NM, int NF>
+hipError_t launch_mortar_mfma_nf(int kind, const FluxParams &p, const double *ops, int64_t nif,
+                                 const double *Uc, const double *Uf, const double *nrm, double *rc,
+                                 double *rf, hipStream_t s)
+{
+    constexpr int W = mortar_waves<NM>();
+    // persistent workgroups: as many as are resident at once, so that each
+    // stages the operators into LDS once
+    static const int cap = [] {
+        int dev = 0, per = 1, cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mortar_mfma_kernel<NM, NF>, 64 * W,
+                                                         0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        return cus * per;
+    }();
+    const int64_t need = (nif + 16 * W - 1) / (16 * W);
+    hipLaunchKernelGGL((mortar_mfma_kernel<NM, NF>), dim3((unsigned)(need < cap ? need : cap)),
+                       dim3(64 * W), 0, s, kind, p, ops, nif, Uc, Uf, nrm, rc, rf);
+    return hipGetLastError();
 }
 
 template <int NM>
@@ -475,28 +531,13 @@ hipError_t launch_mortar_mfma(int kind, const FluxParams &p, int nfine, const do
                               int64_t nif, const double *Uc, const double *Uf, const double *nrm,
                               double *rc, double *rf, hipStream_t s)
 {
-    const int64_t need = (nif + 16 * kMortarWaves - 1) / (16 * kMortarWaves);
-    const dim3 g((unsigned)(need < 4096 ? need : 4096)), b(64 * kMortarWaves);
     switch (nfine) {
-    case 1:
-        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 1>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
-                           rc, rf);
-        break;
-    case 2:
-        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 2>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
-                           rc, rf);
-        break;
-    case 3:
-        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 3>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
-                           rc, rf);
-        break;
-    case 4:
-        hipLaunchKernelGGL((mortar_mfma_kernel<NM, 4>), g, b, 0, s, kind, p, ops, nif, Uc, Uf, nrm,
-                           rc, rf);
-        break;
+    case 1: return launch_mortar_mfma_nf<NM, 1>(kind, p, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 2: return launch_mortar_mfma_nf<NM, 2>(kind, p, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 3: return launch_mortar_mfma_nf<NM, 3>(kind, p, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 4: return launch_mortar_mfma_nf<NM, 4>(kind, p, ops, nif, Uc, Uf, nrm, rc, rf, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 // verify_mortar_conservation (src/dg/mortar.cpp:386-413), one interface per
@@ -570,10 +611,11 @@ hipError_t launch_mortar(int kind, const FluxParams &p, int nm, int nfine, const
 {
     if (nif <= 0)
         return hipSuccess;
-    switch (nm) { // orders 1-3 on the matrix cores
+    switch (nm) { // orders 1-4 on the matrix cores
     case 4: return launch_mortar_mfma<4>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
     case 9: return launch_mortar_mfma<9>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
     case 16: return launch_mortar_mfma<16>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
+    case 25: return launch_mortar_mfma<25>(kind, p, nfine, ops, nif, Uc, Uf, nrm, rc, rf, s);
     default: break;
     }
     const int ipb = 256 / nm;

@@ -168,18 +168,20 @@ def test_mortar_large_batch_stride(torch, orc):
 
 
 @pytest.mark.parametrize("n_if", [0, 1, 15, 16, 17])
-def test_mortar_small_counts(torch, orc, n_if):
-    """counts around one wavefront's 16 interfaces (and none)"""
+@pytest.mark.parametrize("order", [3, 4])
+def test_mortar_small_counts(torch, orc, order, n_if):
+    """counts around one wavefront's 16 interfaces (and none), with one
+    (order 3) and two (order 4) 16-row operator tiles"""
     from drifter_amd import interface as itf
 
-    order, ctype, kind = 3, 5, 6
+    ctype, kind = 5, 6
     ms = itf.MortarSpace(order, ctype, device=0)
     nm, nfine = ms.num_dofs(), ms.num_fine_elements()
-    rng = np.random.default_rng(n_if + 3)
+    rng = np.random.default_rng(n_if + 3 + 100 * order)
     Uc, Uf = rng.normal(size=(n_if, nm)), rng.normal(size=(n_if, nfine, nm))
     nrm = _unit_normals(rng, max(n_if, 1), planar=False)[:n_if]
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    print(f"[mortar] small count {n_if}", flush=True)
+    print(f"[mortar] order {order} small count {n_if}", flush=True)
     rc, rf = (t.cpu().numpy() for t in ms.compute_mortar_flux(kind, PARAMS[kind], d(Uc), d(Uf),
                                                                d(nrm)))
     assert rc.shape == (n_if, nm) and rf.shape == (n_if, nfine, nm)
@@ -190,14 +192,17 @@ def test_mortar_small_counts(torch, orc, n_if):
         np.testing.assert_allclose(rf[k], ef, rtol=0, atol=1e-12 * s)
 
 
-def test_mortar_matrix_core_grid_stride(torch, orc):
-    """more interfaces than one pass of the grid (4096 workgroups x 64)"""
+@pytest.mark.parametrize("order,ctype", [(2, 1), (4, 5)])
+def test_mortar_matrix_core_grid_stride(torch, orc, order, ctype):
+    """more interfaces than one pass of the persistent grid (one workgroup
+    per resident slot, 16 interfaces per wavefront: at most 256 CUs x 64
+    wavefronts x 16 = 262,144 interfaces per pass)"""
     from drifter_amd import interface as itf
 
-    order, ctype, kind = 2, 1, 5
+    kind = 5
     ms = itf.MortarSpace(order, ctype, device=0)
     nm, nfine = ms.num_dofs(), ms.num_fine_elements()
-    n_if = 4096 * 64 + 4099
+    n_if = 262144 + 4099
     g = torch.Generator(device="cuda").manual_seed(5)
     Uc = torch.randn((n_if, nm), dtype=torch.float64, device="cuda", generator=g)
     Uf = torch.randn((n_if, nfine, nm), dtype=torch.float64, device="cuda", generator=g)
@@ -206,7 +211,7 @@ def test_mortar_matrix_core_grid_stride(torch, orc):
     print(f"[mortar] grid stride n_if={n_if}", flush=True)
     rc, rf = ms.compute_mortar_flux(kind, PARAMS[kind], Uc, Uf, nrm)
     torch.cuda.synchronize()
-    ks = [0, 262143, 262144, 262160, n_if - 1]
+    ks = [0, 65535, 65536, 81919, 81920, 262143, 262144, n_if - 1]
     Ucn, Ufn, nn = Uc[ks].cpu().numpy(), Uf[ks].cpu().numpy(), nrm[ks].cpu().numpy()
     rcn, rfn = rc[ks].cpu().numpy(), rf[ks].cpu().numpy()
     for i, k in enumerate(ks):
