@@ -30,8 +30,12 @@ template <int N> struct MsCfg {
 
 __device__ __forceinline__ double hinv(double H) { return (H > 1e-10) ? 1.0 / H : 0.0; }
 
+// Four waves per SIMD up to N = 4 (the D rows in registers put N = 4 two
+// VGPRs over 128; the spill sits outside the substep loop and measured
+// 251 us instead of 271 us per step at 40 substeps, 64x64x8 elements).
 template <int N>
 __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
+    __attribute__((amdgpu_waves_per_eu(N <= 4 ? 4 : 1)))
     mode_split_kernel(int64_t E, double dt3, int nsub, double *__restrict__ full,
                       const View<const double> hb, const View<const double> fb, double f0,
                       const Ops ops, const Phys ph)
@@ -47,13 +51,23 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
 
     const int tid = threadIdx.x;
     for (int q = tid; q < n * n; q += blockDim.x)
-        sD[q] = ops.D[q]; // visible after the first barrier, before any use
+        sD[q] = ops.D[q];
     const bool in_tile = tid < C::NT;
     const int le = in_tile ? tid / nh : 0;
     const int ih = in_tile ? tid - le * nh : 0;
     const int64_t e = (int64_t)blockIdx.x * EPB + le;
     const bool act = in_tile && e < E;
     const int a = ih % n, b = ih / n; // 2D fast / slow digit
+    // The two rows of D this thread's stencils use, in registers for all
+    // substeps: the substep loop is bound by LDS bandwidth, and D was a
+    // third of its LDS reads.
+    __syncthreads();
+    double Da[n], Db[n];
+#pragma unroll
+    for (int m = 0; m < n; ++m) {
+        Da[m] = sD[a * n + m];
+        Db[m] = sD[b * n + m];
+    }
 
     double *F = full + (act ? e : 0) * (int64_t)12 * nd;
     double *Hu = F + 0 * nd, *Hv = F + 1 * nd, *eta = F + 2 * nd, *HT = F + 3 * nd,
@@ -124,10 +138,10 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
             double dx = 0.0, dy = 0.0, ex = 0.0, ey = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                dx += sD[a * n + m] * s_a[le][m + n * b];
-                dy += sD[b * n + m] * s_b[le][a + n * m];
-                ex += sD[a * n + m] * s_c[le][m + n * b];
-                ey += sD[b * n + m] * s_c[le][a + n * m];
+                dx += Da[m] * s_a[le][m + n * b];
+                dy += Db[m] * s_b[le][a + n * m];
+                ex += Da[m] * s_c[le][m + n * b];
+                ey += Db[m] * s_c[le][a + n * m];
             }
             de0 = -(dx + dy);
             dU0 = -g * Hb * ex + fh * HVb + 0.0;
@@ -146,10 +160,10 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
             double dx = 0.0, dy = 0.0, ex = 0.0, ey = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                dx += sD[a * n + m] * t_a[le][m + n * b];
-                dy += sD[b * n + m] * t_b[le][a + n * m];
-                ex += sD[a * n + m] * t_c[le][m + n * b];
-                ey += sD[b * n + m] * t_c[le][a + n * m];
+                dx += Da[m] * t_a[le][m + n * b];
+                dy += Db[m] * t_b[le][a + n * m];
+                ex += Da[m] * t_c[le][m + n * b];
+                ey += Db[m] * t_c[le][a + n * m];
             }
             const double de1 = -(dx + dy);
             const double dU1 = -g * Hs * ex + fh * HVs + 0.0;
@@ -208,8 +222,8 @@ __global__ void __launch_bounds__(MsCfg<N>::BLOCK)
         double dx = 0.0, dy = 0.0;
 #pragma unroll
         for (int m = 0; m < n; ++m) {
-            dx += sD[a * n + m] * s_a[le][m + n * b];
-            dy += sD[b * n + m] * s_b[le][a + n * m];
+            dx += Da[m] * s_a[le][m + n * b];
+            dy += Db[m] * s_b[le][a + n * m];
         }
         const double div = dx + dy;
         const double hi = hinv(Hn); // H(ih * n_vert), :575
