@@ -388,6 +388,41 @@ def measure_amr(stream):
     return out
 
 
+def measure_vertical(stream):
+    """sigma-coordinate metrics and operators and the omega diagnosis on the
+    64x64x8 N = 4 node count (4,096,000 nodes), FP64, random fields."""
+    import torch
+
+    from drifter_amd import sigma, vertical
+
+    E, nd = 32768, 125
+    g = torch.Generator(device="cuda").manual_seed(3)
+    r = lambda: torch.rand((E, nd), dtype=torch.float64, device="cuda", generator=g)
+    eta, h = 0.1 * r(), 20.0 + 100.0 * r()
+    dex, dey, dhx, dhy, det = (1e-3 * r() for _ in range(5))
+    sg = -r()
+    out = {}
+    n = E * nd
+    ms = time_launches(lambda: sigma.sigma_metrics(sg.view(-1), eta.view(-1), h.view(-1),
+                                                   dex.view(-1), dey.view(-1), dhx.view(-1),
+                                                   dhy.view(-1), stream=stream), 20, 3, stream)
+    out["sigma_metrics_4096000_points_us"] = round(ms * 1e3, 1)
+    ops = sigma.SigmaCoordinateOperators(4)
+    ops.update_sigma_metrics(eta, h, dex, dey, dhx, dhy, det, stream=stream)
+    U, dU, u, v, om, wm = (r() for _ in range(6))
+    ms = time_launches(lambda: ops.horizontal_gradient(U, stream), 20, 3, stream)
+    out["sigma_horizontal_gradient_32768_n4_us"] = round(ms * 1e3, 1)
+    ms = time_launches(lambda: ops.material_derivative(U, dU, u, v, om, wm, stream), 20, 3,
+                       stream)
+    out["sigma_material_derivative_32768_n4_us"] = round(ms * 1e3, 1)
+    vv = vertical.VerticalVelocityDiagnosis(4)
+    ms = time_launches(lambda: vv.diagnose_omega(U, h, stream), 20, 3, stream)
+    out["diagnose_omega_32768_n4_us"] = round(ms * 1e3, 1)
+    out["diagnose_omega_gb_s"] = round(n * 24 / (ms * 1e-3) / 1e9, 1)
+    log(f"sigma / vertical: {out}")
+    return out
+
+
 def measure_indicator(stream):
     """Learned indicator: training on 50 seeded coastline states (about 2M
     element/label pairs, 16 epochs), misclassification against the estimator on 5
@@ -520,6 +555,7 @@ def main():
         extras.update(measure_coastline(args, stream))
         extras.update(measure_faces(stream))
         extras.update(measure_amr(stream))
+        extras.update(measure_vertical(stream))
         extras.update(measure_indicator(stream))
 
     cpu = None if args.no_cpu else cpu_baseline(case)

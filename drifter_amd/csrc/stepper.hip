@@ -8,48 +8,62 @@ namespace {
 __device__ __forceinline__ double wave_max(double a, double b) { return a > b ? a : b; }
 
 // Per element: sqrt(9.81 * max H) + max(max|u|, max|v|) on the derived state
-// of U (update_derived, src/physics/primitive_equations.cpp:21-28).  One wave
-// per element; partial[block] = max over the block's elements.
-template <int WAVES>
-__global__ void __launch_bounds__(64 * WAVES)
+// of U (update_derived, src/physics/primitive_equations.cpp:21-28).  One
+// thread per grid line (e, L) of n nodes: in the line layout consecutive
+// lines of a plane are consecutive addresses, so a wavefront's loads are
+// contiguous.  The n^2 line maxima of an element meet in LDS, where one
+// thread per element forms its wave speed; partial[block] = max over the
+// block's elements.  max and sqrt are exact, so any grouping gives the
+// reference's value.
+__global__ void __launch_bounds__(256)
     wave_speed_kernel(int64_t E, int n, View<const double> U, View<const double> h,
                       double *partial)
 {
-    __shared__ double red[WAVES];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nd = n * n * n;
+    __shared__ double lm[3][256];
+    __shared__ double red[4];
+    const int nl = n * n, epb = 256 / nl;
+    const int t = threadIdx.x;
+    const int le = t / nl, L = t - le * nl;
     double cmax = 0.0;
-    for (int64_t e = (int64_t)blockIdx.x * WAVES + w; e < E; e += (int64_t)gridDim.x * WAVES) {
-        double hm = -INFINITY, um = 0.0, vm = 0.0;
-        for (int d = lane; d < nd; d += 64) {
-            const int L = d / n, i = d - L * n;
-            const int64_t o = node_offset(U, e, L, i);
-            const double eta = U.p[2 * U.fs + o];
-            const double H = eta + h.p[node_offset(h, e, L, i)];
-            const double hi = (H > 1e-10) ? 1.0 / H : 0.0;
-            const double u = U.p[0 * U.fs + o] * hi;
-            const double v = U.p[1 * U.fs + o] * hi;
-            hm = wave_max(hm, H);
-            um = wave_max(um, fabs(u));
-            vm = wave_max(vm, fabs(v));
+    const int64_t nbatch = (E + epb - 1) / epb;
+    for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+        const int64_t e = b * epb + le;
+        if (le < epb && e < E) {
+            double hm = -INFINITY, um = 0.0, vm = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const int64_t o = node_offset(U, e, L, i);
+                const double eta = U.p[2 * U.fs + o];
+                const double H = eta + h.p[node_offset(h, e, L, i)];
+                const double hi = (H > 1e-10) ? 1.0 / H : 0.0;
+                const double u = U.p[0 * U.fs + o] * hi;
+                const double v = U.p[1 * U.fs + o] * hi;
+                hm = wave_max(hm, H);
+                um = wave_max(um, fabs(u));
+                vm = wave_max(vm, fabs(v));
+            }
+            lm[0][t] = hm;
+            lm[1][t] = um;
+            lm[2][t] = vm;
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            hm = wave_max(hm, __shfl_xor(hm, o));
-            um = wave_max(um, __shfl_xor(um, o));
-            vm = wave_max(vm, __shfl_xor(vm, o));
+        __syncthreads();
+        if (t < epb && b * epb + t < E) {
+            double hm = -INFINITY, um = 0.0, vm = 0.0;
+            for (int q = t * nl; q < (t + 1) * nl; ++q) {
+                hm = wave_max(hm, lm[0][q]);
+                um = wave_max(um, lm[1][q]);
+                vm = wave_max(vm, lm[2][q]);
+            }
+            cmax = wave_max(cmax, sqrt(9.81 * hm) + wave_max(um, vm));
         }
-        const double ce = sqrt(9.81 * hm) + wave_max(um, vm);
-        cmax = wave_max(cmax, ce);
+        __syncthreads(); // lm is rewritten by the next batch
     }
-    if (lane == 0)
-        red[w] = cmax;
+    for (int o = 32; o > 0; o >>= 1)
+        cmax = wave_max(cmax, __shfl_xor(cmax, o));
+    if ((t & 63) == 0)
+        red[t >> 6] = cmax;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int i = 1; i < WAVES; ++i)
-            m = wave_max(m, red[i]);
-        partial[blockIdx.x] = m;
-    }
+    if (t == 0)
+        partial[blockIdx.x] = wave_max(wave_max(red[0], red[1]), wave_max(red[2], red[3]));
 }
 
 template <int WAVES>
@@ -132,7 +146,9 @@ hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, View<co
                              double *partial, int64_t npart, hipStream_t s)
 {
     const int n = order + 1;
-    hipLaunchKernelGGL(wave_speed_kernel<4>, dim3((unsigned)npart), dim3(256), 0, s, E, n, U, h,
+    if (n * n > 256)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(wave_speed_kernel, dim3((unsigned)npart), dim3(256), 0, s, E, n, U, h,
                        partial);
     return hipGetLastError();
 }

@@ -19,54 +19,128 @@ struct Levels {
 };
 
 // diagnose_omega (:55-97), with div = A (+ B: diagnose_omega_from_velocity,
-// :99-108) formed first
+// :99-108) formed first.  The column length N is a template argument, so the
+// loop unrolls and the node coordinates x.s[k] are read with constant
+// indices (kernel-argument registers): indexed at run time they compiled to
+// memory loads in every iteration and the kernel ran at 2 TB/s.
+template <int N, bool HAS_B>
 __global__ void __launch_bounds__(256)
-    omega_kernel(int n, int64_t ncol_total, Levels x, const double *__restrict__ A,
+    omega_kernel(int64_t ncol_total, Levels x, const double *__restrict__ A,
                  const double *__restrict__ B, const double *__restrict__ H, int64_t es,
                  double *__restrict__ omega)
 {
-    const int n2 = n * n;
+    constexpr int n2 = N * N;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncol_total;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e = c / n2;
         const int i = (int)(c - e * n2);
-        const int64_t o = e * es + (int64_t)i * n;
-        double dp = B ? A[o] + B[o] : A[o], hp = H[o];
-        omega[o] = 0.0; // omega(-1) = 0
-        double integral = 0.0;
-        for (int k = 1; k < n; ++k) {
-            const double dc = B ? A[o + k] + B[o + k] : A[o + k], hc = H[o + k];
-            const double H_avg = 0.5 * (hp + hc);
-            const double dsigma = (x.s[k] - x.s[k - 1]) * 0.5;
-            const double div_avg = 0.5 * (dp + dc);
-            integral += div_avg * dsigma;
-            omega[o + k] = (H_avg > 1e-10) ? -integral / H_avg : 0.0;
-            dp = dc;
-            hp = hc;
+        const int64_t o = e * es + (int64_t)i * N;
+        double dv[N], hv[N], ov[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            dv[k] = HAS_B ? A[o + k] + B[o + k] : A[o + k];
+            hv[k] = H[o + k];
         }
+        ov[0] = 0.0; // omega(-1) = 0
+        double integral = 0.0;
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+            const double H_avg = 0.5 * (hv[k - 1] + hv[k]);
+            const double dsigma = (x.s[k] - x.s[k - 1]) * 0.5;
+            const double div_avg = 0.5 * (dv[k - 1] + dv[k]);
+            integral += div_avg * dsigma;
+            ov[k] = (H_avg > 1e-10) ? -integral / H_avg : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            omega[o + k] = ov[k];
     }
 }
 
-// ColumnVerticalVelocity::diagnose_column (:246-275), columns of n levels
+// ColumnVerticalVelocity::diagnose_column (:246-275), columns of N levels
+template <int N>
 __global__ void __launch_bounds__(256)
-    column_kernel(int n, int64_t ncol, Levels sg, const double *__restrict__ div,
+    column_kernel(int64_t ncol, Levels sg, const double *__restrict__ div,
                   const double *__restrict__ H, const double *__restrict__ ob,
                   double *__restrict__ omega)
 {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncol;
          c += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t o = c * n;
-        const double b = ob ? ob[c] : 0.0;
-        omega[o] = b;
-        double integral = 0.0;
-        for (int k = 1; k < n; ++k) {
-            const double dsigma = sg.s[k] - sg.s[k - 1];
-            const double div_avg = 0.5 * (div[o + k - 1] + div[o + k]);
-            integral += div_avg * dsigma;
-            const double H_avg = 0.5 * (H[o + k - 1] + H[o + k]);
-            omega[o + k] = (H_avg > 1e-10) ? b - integral / H_avg : b;
+        const int64_t o = c * N;
+        double dv[N], hv[N], ov[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            dv[k] = div[o + k];
+            hv[k] = H[o + k];
         }
+        const double b = ob ? ob[c] : 0.0;
+        ov[0] = b;
+        double integral = 0.0;
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+            const double dsigma = sg.s[k] - sg.s[k - 1];
+            const double div_avg = 0.5 * (dv[k - 1] + dv[k]);
+            integral += div_avg * dsigma;
+            const double H_avg = 0.5 * (hv[k - 1] + hv[k]);
+            ov[k] = (H_avg > 1e-10) ? b - integral / H_avg : b;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            omega[o + k] = ov[k];
     }
+}
+
+unsigned grid_for(int64_t items)
+{
+    const int64_t b = (items + 255) / 256;
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+
+template <int N>
+void launch_omega_n(int64_t ncol, const Levels &x, const double *A, const double *B,
+                    const double *H, int64_t es, double *omega, hipStream_t s)
+{
+    if (B)
+        hipLaunchKernelGGL((omega_kernel<N, true>), dim3(grid_for(ncol)), dim3(256), 0, s, ncol, x,
+                           A, B, H, es, omega);
+    else
+        hipLaunchKernelGGL((omega_kernel<N, false>), dim3(grid_for(ncol)), dim3(256), 0, s, ncol,
+                           x, A, B, H, es, omega);
+}
+
+hipError_t launch_omega(int n, int64_t ncol, const Levels &x, const double *A, const double *B,
+                        const double *H, int64_t es, double *omega, hipStream_t s)
+{
+    switch (n) {
+    case 2: launch_omega_n<2>(ncol, x, A, B, H, es, omega, s); break;
+    case 3: launch_omega_n<3>(ncol, x, A, B, H, es, omega, s); break;
+    case 4: launch_omega_n<4>(ncol, x, A, B, H, es, omega, s); break;
+    case 5: launch_omega_n<5>(ncol, x, A, B, H, es, omega, s); break;
+    case 6: launch_omega_n<6>(ncol, x, A, B, H, es, omega, s); break;
+    case 7: launch_omega_n<7>(ncol, x, A, B, H, es, omega, s); break;
+    case 8: launch_omega_n<8>(ncol, x, A, B, H, es, omega, s); break;
+    case 9: launch_omega_n<9>(ncol, x, A, B, H, es, omega, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_column(int n, int64_t ncol, const Levels &sg, const double *div,
+                         const double *H, const double *ob, double *omega, hipStream_t s)
+{
+    const dim3 g(grid_for(ncol)), b(256);
+    switch (n) {
+    case 2: hipLaunchKernelGGL(column_kernel<2>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 3: hipLaunchKernelGGL(column_kernel<3>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 4: hipLaunchKernelGGL(column_kernel<4>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 5: hipLaunchKernelGGL(column_kernel<5>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 6: hipLaunchKernelGGL(column_kernel<6>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 7: hipLaunchKernelGGL(column_kernel<7>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 8: hipLaunchKernelGGL(column_kernel<8>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    case 9: hipLaunchKernelGGL(column_kernel<9>, g, b, 0, s, ncol, sg, div, H, ob, omega); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // omega_to_physical_w (:110-121) / omega_to_w_simple (:123-143)
@@ -105,12 +179,6 @@ __global__ void __launch_bounds__(256)
         }
         out[e] = sqrt(r2 / nd);
     }
-}
-
-unsigned grid_for(int64_t items)
-{
-    const int64_t b = (items + 255) / 256;
-    return (unsigned)(b < 65536 ? b : 65536);
 }
 
 // ColumnVerticalVelocity(order) sigma nodes (:180-244): hard-coded LGL
@@ -169,10 +237,8 @@ drf_status omega_common(const char *who, int order, int64_t E, const void *A, co
     double w[kMaxN];
     DRF_REQUIRE(lgl_nodes(n, x.s, w) == 0, DRF_ERR_ARG, std::string(who) + ": LGL nodes failed");
     const int64_t ncol = E * n * n;
-    hipLaunchKernelGGL(omega_kernel, dim3(grid_for(ncol)), dim3(256), 0, (hipStream_t)stream, n,
-                       ncol, x, (const double *)A, (const double *)B, (const double *)H, es,
-                       (double *)omega);
-    DRF_HIP_CHECK(hipGetLastError());
+    DRF_HIP_CHECK(launch_omega(n, ncol, x, (const double *)A, (const double *)B,
+                               (const double *)H, es, (double *)omega, (hipStream_t)stream));
     return DRF_OK;
 }
 
@@ -261,10 +327,9 @@ drf_status drf_diagnose_column_dev(int order, int64_t ncol, const void *div, con
     if (ncol == 0)
         return DRF_OK;
     DRF_REQUIRE(div && H && omega, DRF_ERR_ARG, "drf_diagnose_column_dev: null pointer");
-    hipLaunchKernelGGL(column_kernel, dim3(grid_for(ncol)), dim3(256), 0, (hipStream_t)stream,
-                       order + 1, ncol, sg, (const double *)div, (const double *)H,
-                       (const double *)omega_bottom, (double *)omega);
-    DRF_HIP_CHECK(hipGetLastError());
+    DRF_HIP_CHECK(launch_column(order + 1, ncol, sg, (const double *)div, (const double *)H,
+                                (const double *)omega_bottom, (double *)omega,
+                                (hipStream_t)stream));
     return DRF_OK;
 }
 

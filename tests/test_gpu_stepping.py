@@ -94,6 +94,40 @@ def test_wave_speed_bit_exact(orc):
         s.close()
 
 
+@pytest.mark.parametrize("order", range(1, 9))
+def test_wave_speed_every_order_and_ragged_batches(orc, order):
+    """one thread per grid line, n^2 lines per element: element counts that
+    leave the last batch of a block part-filled, every order"""
+    import torch
+
+    case = synth.make_case(5, 3, 3, order, 100000.0, "shelf")  # 45 elements
+    H, u, v, _, _ = orc.derived(case.order, case.U, case.h)
+    ref = orc.max_wave_speed(case.order, H, u, v)
+    s = PrimitiveEquationsSolver(case.order)
+    s.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    print(f"[wave speed] order={order} E={case.E}", flush=True)
+    Ud = torch.from_numpy(synth.to_soa(case.U, case.E, case.nd)).cuda()
+    assert s.max_wave_speed_dev(Ud, _lib.LAYOUT_SOA) == ref
+    Ul = torch.from_numpy(synth.to_lines(case.U, case.E, case.n)).cuda()
+    assert s.max_wave_speed_dev(Ul, _lib.LAYOUT_LINES) == ref
+    s.close()
+
+
+def test_wave_speed_grid_stride(orc):
+    """64x64x8 at N = 4: more element batches (3,277) than blocks (2,048)"""
+    import torch
+
+    case = synth.config2()
+    H, u, v, _, _ = orc.derived(case.order, case.U, case.h)
+    ref = orc.max_wave_speed(case.order, H, u, v)
+    s = PrimitiveEquationsSolver(4)
+    s.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    Ul = torch.from_numpy(synth.to_lines(case.U, case.E, case.n)).cuda()
+    print(f"[wave speed] grid stride E={case.E}", flush=True)
+    assert s.max_wave_speed_dev(Ul, _lib.LAYOUT_LINES) == ref
+    s.close()
+
+
 @pytest.mark.parametrize("order,mesh", [(3, (8, 8, 4)), (4, (8, 8, 2))])
 def test_mode_split_matches_oracle(orc, order, mesh):
     import torch
