@@ -167,7 +167,7 @@ def measure_extras(args, case, s64, U, r64, stream, tune=None):
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
     R32 = torch.empty_like(U32)
-    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=5, stream=stream)
+    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=8, stream=stream)
     ms32_warm = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
                               args.steps, args.warmup, stream)
     err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
@@ -271,7 +271,7 @@ def measure_coastline(args, stream):
         s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
         U = torch.from_numpy(synth.to_lines(case.U, E, 4)).to("cuda").to(dt)
         R = torch.empty_like(U)
-        s.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
+        s.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
         # 8 sets (0.9-1.7 GB): state from HBM every launch, as for the value
         ms = time_rotating(s, U, R, 8, args.steps, args.warmup, stream)
         out[f"coastline_39280_n3_{name}_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
@@ -529,7 +529,7 @@ def main():
         tune = (wpc, bool(remap), float("nan"))
         log(f"launch shape fixed: {wpc} workgroups/CU, xcd_remap={bool(remap)}")
     else:
-        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=5, stream=stream)
+        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
         log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
     # The same input and output every launch: 361 MB, of which the 256 MB
     # last-level cache keeps a part between launches.  Reported beside the

@@ -701,23 +701,48 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
         (void)hipEventDestroy(e0);
         return fail(DRF_ERR_HIP, "hipEventCreate failed");
     }
+    // The trials cycle over the caller's state and output and kNset - 1
+    // temporary copies (1.3 GB at 64x64x8, N = 4, FP64), so that every timed
+    // launch reads its state from HBM, as a solver's stages do, rather than
+    // from what the previous launch left in the 256 MB last-level cache.  If
+    // the copies cannot be allocated, the trials run on the caller's buffers.
+    constexpr int kNset = 4;
+    const size_t bytes = elem_size(c) * (size_t)state_len(c);
+    const void *Us[kNset] = {U};
+    void *Rs[kNset] = {dUdt};
+    int nset = 1;
+    drf_status res = DRF_OK;
+    for (; nset < kNset; ++nset) {
+        void *u = nullptr, *r = nullptr;
+        if (hipMalloc(&u, bytes) != hipSuccess || hipMalloc(&r, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            if (u)
+                (void)hipFree(u);
+            break;
+        }
+        Us[nset] = u;
+        Rs[nset] = r;
+        if (hipMemcpyAsync(u, U, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            res = fail(DRF_ERR_HIP, "hipMemcpyAsync failed");
+            ++nset;
+            break;
+        }
+    }
     // resident workgroups per CU (values above the occupancy limit clamp to
     // it) x XCD-aware batch order
     static const int kWpc[] = {1, 2, 3, 4, 5, 6, 8};
     float best = 1e30f;
     int bw = c->wg_per_cu, br = c->xcd_remap;
-    drf_status res = DRF_OK;
     for (int remap = 0; remap < 2 && res == DRF_OK; ++remap)
         for (int wpc : kWpc) {
             c->wg_per_cu = wpc;
             c->xcd_remap = remap;
-            res = rhs_into(c, U, dUdt, layout, s); // warm-up
-            for (int r = 0; r < reps && res == DRF_OK; ++r) {
-                if (r == 0 && hipEventRecord(e0, s) != hipSuccess)
-                    res = fail(DRF_ERR_HIP, "hipEventRecord failed");
-                if (res == DRF_OK)
-                    res = rhs_into(c, U, dUdt, layout, s);
-            }
+            for (int k = 0; k < nset && res == DRF_OK; ++k) // warm-up
+                res = rhs_into(c, Us[k], Rs[k], layout, s);
+            if (res == DRF_OK && hipEventRecord(e0, s) != hipSuccess)
+                res = fail(DRF_ERR_HIP, "hipEventRecord failed");
+            for (int r = 0; r < reps && res == DRF_OK; ++r)
+                res = rhs_into(c, Us[r % nset], Rs[r % nset], layout, s);
             float ms = 0.f;
             if (res == DRF_OK && (hipEventRecord(e1, s) != hipSuccess ||
                                   hipEventSynchronize(e1) != hipSuccess ||
@@ -732,6 +757,11 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
                 br = remap;
             }
         }
+    (void)hipStreamSynchronize(s);
+    for (int k = 1; k < nset; ++k) {
+        (void)hipFree(const_cast<void *>(Us[k]));
+        (void)hipFree(Rs[k]);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->wg_per_cu = bw;
