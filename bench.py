@@ -388,6 +388,72 @@ def measure_amr(stream):
     return out
 
 
+def measure_adapt_cycle(stream):
+    """One adaptation cycle of the 64x64x8 N = 4 octree mesh, stage by stage
+    (wall clock, each stage synchronised): the gradient estimator and its
+    marking on the device, the refine list on the host, octree refine +
+    balance and the face connectivity on the host, the transfer map on the
+    host, and the transfer of the five state fields and the bathymetry to the
+    refined mesh on the device."""
+    import numpy as np
+    import torch
+
+    from drifter_amd import adapt, amr, synth
+    from drifter_amd.mesh import OctreeAdapter
+
+    order, nd, L = 4, 125, 100000.0
+    m = OctreeAdapter(0.0, L, 0.0, L, -1.0, 0.0)
+    m.build_uniform(64, 64, 8)
+    old_b, levels, _ = m.elements()
+    case = synth.make_case_on(old_b, order, L, "shelf")
+    E = old_b.shape[0]
+    U = torch.from_numpy(case.U.reshape(E, 5 * nd)).to("cuda")
+    h = torch.from_numpy(case.h.reshape(E, nd)).to("cuda")
+    hc = torch.from_numpy(adapt.element_char_length(old_b)).to("cuda")
+    lv = torch.from_numpy(np.ascontiguousarray(levels, dtype=np.int32)).to("cuda")
+    est = adapt.GradientErrorEstimator(order, refine_threshold=0.5)
+    eta = U.view(-1)[2 * nd:]  # field 2 of every element, element stride 5 nd
+    est.estimate(eta, hc, lv, elem_stride=5 * nd, stream=stream)  # warm-up
+    torch.cuda.synchronize()
+    t = {}
+    t0 = time.perf_counter()
+    r = est.estimate(eta, hc, lv, elem_stride=5 * nd, stream=stream)
+    torch.cuda.synchronize()
+    t["estimate_and_mark_device"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    act = r["action"].cpu().numpy()
+    els = np.nonzero(act == 1)[0]
+    t["refine_list_to_host"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.refine(els, [7] * els.size)
+    m.balance()
+    t["octree_refine_balance_host"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.build_face_connections()
+    t["face_connections_host"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    new_b, _, _ = m.elements()
+    src, boxes = amr.refine_transfer_map(old_b, new_b)
+    src_d, boxes_d = torch.from_numpy(src).to("cuda"), torch.from_numpy(boxes).to("cuda")
+    t["transfer_map_host"] = time.perf_counter() - t0
+    def transfer():
+        return [amr.interpolate_boxes(order, U[:, f * nd:(f + 1) * nd], src_d, boxes_d, stream)
+                for f in range(5)] + [amr.interpolate_boxes(order, h, src_d, boxes_d, stream)]
+
+    transfer()  # the first call's output allocations are not the transfer's cost
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fields = transfer()
+    torch.cuda.synchronize()
+    t["transfer_device"] = time.perf_counter() - t0
+    out = {"adapt_cycle_elements_before_after": [int(E), int(new_b.shape[0])],
+           "adapt_cycle_refined": int(els.size)}
+    out.update({f"adapt_cycle_{k}_ms": round(v * 1e3, 3) for k, v in t.items()})
+    del fields
+    log(f"adapt cycle: {out}")
+    return out
+
+
 def measure_vertical(stream):
     """sigma-coordinate metrics and operators and the omega diagnosis on the
     64x64x8 N = 4 node count (4,096,000 nodes), FP64, random fields."""
@@ -556,6 +622,7 @@ def main():
         extras.update(measure_faces(stream))
         extras.update(measure_amr(stream))
         extras.update(measure_vertical(stream))
+        extras.update(measure_adapt_cycle(stream))
         extras.update(measure_indicator(stream))
 
     cpu = None if args.no_cpu else cpu_baseline(case)

@@ -217,3 +217,58 @@ def test_large_round_trip(torch, order):
     assert err <= TOL * U.abs().max().item() * 10, err
     allp = amr.transfer_solution(order, U, torch.arange(P, device="cuda"))
     assert torch.equal(allp.view(P, 8, nd), ch)
+
+
+def _lagrange_matrix(x, lo, hi):
+    """M[e][p][i] = L_i(lo_e + (x_p + 1) (hi_e - lo_e) / 2) on nodes x."""
+    xi = lo[:, None] + 0.5 * (x[None, :] + 1.0) * (hi - lo)[:, None]  # (E, n)
+    n = x.size
+    M = np.ones((lo.size, n, n))
+    for i in range(n):
+        for j in range(n):
+            if j != i:
+                M[:, :, i] *= (xi - x[j]) / (x[i] - x[j])
+    return M
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_interpolate_boxes_any_boxes_invalid_sources_grid_stride(torch, orc, order):
+    """Sub-boxes mixing whole intervals, the two halves and arbitrary
+    intervals; sources out of range give NaN rows; more outputs than one pass
+    of the grid (4,096 workgroups)."""
+    from drifter_amd import amr
+
+    n = order + 1
+    nd = n ** 3
+    epb = max(1024 // nd, 1)
+    n_src, n_out = 300, 4096 * epb + 777
+    rng = np.random.default_rng(order)
+    U = rng.standard_normal((n_src, nd))
+    src = rng.integers(0, n_src, n_out)
+    src[[5, 17]] = -1
+    src[[6, 4096 * epb + 3]] = n_src
+    boxes = np.empty((n_out, 6))
+    for a in range(3):
+        kind = rng.integers(0, 4, n_out)
+        lo = np.where(kind == 0, -1.0, np.where(kind == 1, -1.0, 0.0))
+        hi = np.where(kind == 0, 1.0, np.where(kind == 1, 0.0, 1.0))
+        r = np.sort(rng.uniform(-1, 1, (n_out, 2)), axis=1)
+        lo = np.where(kind == 3, r[:, 0], lo)
+        hi = np.where(kind == 3, r[:, 1], hi)
+        boxes[:, 2 * a], boxes[:, 2 * a + 1] = lo, hi
+    print(f"[interpolate_boxes] order={order} n_out={n_out}", flush=True)
+    got = amr.interpolate_boxes(order, torch.from_numpy(U).cuda(), torch.from_numpy(src).cuda(),
+                                torch.from_numpy(boxes).cuda()).cpu().numpy()
+    bad = (src < 0) | (src >= n_src)
+    assert np.all(np.isnan(got[bad]))
+    x, _ = orc.lgl(n)
+    ok = ~bad
+    Us = U[src[ok]].reshape(-1, n, n, n)  # [e][k][j][i]
+    Mx, My, Mz = (_lagrange_matrix(x, boxes[ok, 2 * a], boxes[ok, 2 * a + 1]) for a in range(3))
+    t1 = np.einsum("epi,ekji->ekjp", Mx, Us)
+    t2 = np.einsum("eqj,ekjp->ekqp", My, t1)
+    want = np.einsum("erk,ekqp->erqp", Mz, t2).reshape(-1, nd)
+    scale = np.abs(U).max()
+    assert np.abs(got[ok] - want).max() <= 1e-12 * scale
+    whole = ok & np.all(boxes == np.array([-1, 1, -1, 1, -1, 1.0]), axis=1)
+    np.testing.assert_array_equal(got[whole], U[src[whole]])
