@@ -425,8 +425,7 @@ def measure_adapt_cycle(stream):
     els = np.nonzero(act == 1)[0]
     t["refine_list_to_host"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    m.refine(els, [7] * els.size)
-    m.balance()
+    m.refine(els, [7] * els.size)  # refines, then balances
     t["octree_refine_balance_host"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     m.build_face_connections()

@@ -32,6 +32,7 @@ struct drf_mesh {
         int64_t child0;   // first child (children are contiguous), -1 for a leaf
         int nchild;
         int64_t leaf;     // element number, -1 if not a leaf
+        int64_t parent;   // -1 for the root
     };
     double dom[6];
     std::vector<Node> nodes;
@@ -87,6 +88,7 @@ void create_children(drf_mesh &m, int64_t id)
                 c.b[5] = p.b[4] + (iz + 1) * dz;
                 c.child0 = -1;
                 c.leaf = -1;
+                c.parent = id;
                 m.nodes.push_back(c);
             }
     m.nodes[id].child0 = first;
@@ -130,12 +132,24 @@ void across(const double *b, int f, double *p)
     p[a] = (f % 2) ? hi + 0.5 * (hi - lo) : lo - 0.5 * (hi - lo);
 }
 
-// deepest node containing the point, or -1 outside the domain
-int64_t locate(const drf_mesh &m, const double *p)
+// The deepest node containing the centre p of the same-size box across
+// face f of node id (the first child containing p at every level, as the
+// reference descends from the root), reached from the nearest ancestor of id
+// that contains p instead of from the root.  p lies half a
+// cell of id's levels off every grid line of those levels, so it is strictly
+// inside that ancestor and every node above it: the root's descent passes
+// through the ancestor, and below it both descents take the same children.
+int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
 {
+    double p[3];
+    across(m.nodes[id].b, f, p);
     if (!inside(m.dom, p))
         return -1;
-    int64_t cur = 0;
+    int64_t cur = m.nodes[id].parent;
+    while (cur > 0 && !inside(m.nodes[cur].b, p))
+        cur = m.nodes[cur].parent;
+    if (cur < 0)
+        cur = 0;
     while (m.nodes[cur].nchild) {
         const Node &nd = m.nodes[cur];
         int64_t next = -1;
@@ -151,19 +165,19 @@ int64_t locate(const drf_mesh &m, const double *p)
     return cur;
 }
 
-int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
-{
-    double p[3];
-    across(m.nodes[id].b, f, p);
-    return locate(m, p);
-}
-
+// 2:1 balance, pass by pass over the leaves in leaf order, splitting a
+// neighbour more than one level coarser on the axes that need it.  A leaf
+// that existed when the previous pass began cannot force a split in a later
+// pass (its neighbours only became finer), so passes after the first visit
+// only the leaves created during the previous pass, still in leaf order: the
+// same splits in the same order as rescanning every leaf.
 void balance(drf_mesh &m)
 {
     bool changed = true;
+    std::vector<int64_t> snapshot = m.leaves;
     while (changed) {
         changed = false;
-        const std::vector<int64_t> snapshot = m.leaves;
+        const int64_t first_new = (int64_t)m.nodes.size();
         for (int64_t id : snapshot)
             for (int f = 0; f < 6; ++f) {
                 const int64_t nb = neighbour_node(m, id, f);
@@ -179,8 +193,13 @@ void balance(drf_mesh &m)
                     changed = true;
                 }
             }
-        if (changed)
+        if (changed) {
             number_leaves(m);
+            snapshot.clear();
+            for (int64_t id : m.leaves)
+                if (id >= first_new)
+                    snapshot.push_back(id);
+        }
     }
 }
 
@@ -215,6 +234,7 @@ drf_status drf_mesh_create(double xmin, double xmax, double ymin, double ymax, d
     Node root{};
     std::memcpy(root.b, d, sizeof(d));
     root.child0 = -1;
+    root.parent = -1;
     root.mask = 7;
     m->nodes.push_back(root);
     number_leaves(*m);
@@ -235,6 +255,7 @@ drf_status drf_mesh_build_uniform(drf_mesh *m, int nx, int ny, int nz)
             ++lv[a];
     Node root = m->nodes[0];
     root.child0 = -1;
+    root.parent = -1;
     root.nchild = 0;
     root.mask = 7;
     root.lev[0] = root.lev[1] = root.lev[2] = 0;
