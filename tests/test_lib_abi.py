@@ -85,3 +85,26 @@ def test_basis_export():
     assert lib.drf_basis(4, _lib.dptr(x), _lib.dptr(w), _lib.dptr(D)) == 0
     assert x[0] == -1.0 and x[-1] == 1.0 and abs(w.sum() - 2) < 1e-15
     assert lib.drf_basis(0, None, None, None) == _lib.DRF_ERR_ARG
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU fallback: without the built HIP library the package raises
+    instead of computing anything (checked in a fresh interpreter so the
+    loaded library of this process is not touched)."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from drifter_amd import _lib\n"
+        "_lib.LIB_PATH = %r\n"
+        "try:\n"
+        "    _lib.load()\n"
+        "except RuntimeError as e:\n"
+        "    print('raised:', 'no CPU fallback' in str(e))\n"
+        "    sys.exit(0)\n"
+        "sys.exit(1)\n"
+    ) % (str(ROOT), str(tmp_path / "libdrifter_hip.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "raised: True" in r.stdout
