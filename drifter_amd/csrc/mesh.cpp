@@ -16,9 +16,11 @@
 //   - a face is a boundary face when it lies within 1e-12 of the domain.
 // Storage differs from the reference's pointer tree: nodes live in one
 // vector with contiguous children, and traversals use explicit stacks/queues.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "drf_internal.hpp"
@@ -165,12 +167,55 @@ int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
     return cur;
 }
 
+// fn(i0, i1) over [0, n) in contiguous chunks on up to 16 threads (one chunk
+// below 4,096 items); fn must only read shared state and write its own range.
+template <class F> void parallel_chunks(int64_t n, F fn)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t nt = std::min<int64_t>({n / 4096 + 1, hw ? (int64_t)hw : 1, 16});
+    if (nt <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int64_t t = 1; t < nt; ++t) {
+        const int64_t i0 = std::min(n, t * chunk), i1 = std::min(n, (t + 1) * chunk);
+        try {
+            pool.emplace_back([=] { fn(i0, i1); });
+        } catch (...) { // no thread available: this chunk runs here
+            fn(i0, i1);
+        }
+    }
+    fn((int64_t)0, std::min(n, chunk));
+    for (auto &th : pool)
+        th.join();
+}
+
+// Does leaf id have a face neighbour more than one level coarser on some axis?
+bool needs_split(const drf_mesh &m, int64_t id)
+{
+    for (int f = 0; f < 6; ++f) {
+        const int64_t nb = neighbour_node(m, id, f);
+        if (nb < 0 || m.nodes[nb].nchild)
+            continue;
+        for (int a = 0; a < 3; ++a)
+            if (m.nodes[id].lev[a] - m.nodes[nb].lev[a] > 1)
+                return true;
+    }
+    return false;
+}
+
 // 2:1 balance, pass by pass over the leaves in leaf order, splitting a
 // neighbour more than one level coarser on the axes that need it.  A leaf
 // that existed when the previous pass began cannot force a split in a later
 // pass (its neighbours only became finer), so passes after the first visit
 // only the leaves created during the previous pass, still in leaf order: the
-// same splits in the same order as rescanning every leaf.
+// same splits in the same order as rescanning every leaf.  Within a pass, a
+// leaf with no such neighbour when the pass begins gets none later in the
+// pass (splits only make neighbours finer), so the leaves are first filtered
+// in parallel against the tree as it stands, and only those flagged are
+// checked, in order, against the tree as the pass changes it.
 void balance(drf_mesh &m)
 {
     bool changed = true;
@@ -178,7 +223,16 @@ void balance(drf_mesh &m)
     while (changed) {
         changed = false;
         const int64_t first_new = (int64_t)m.nodes.size();
-        for (int64_t id : snapshot)
+        std::vector<char> flag(snapshot.size());
+        parallel_chunks((int64_t)snapshot.size(), [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i)
+                flag[i] = needs_split(m, snapshot[i]);
+        });
+        std::vector<int64_t> work;
+        for (size_t i = 0; i < snapshot.size(); ++i)
+            if (flag[i])
+                work.push_back(snapshot[i]);
+        for (int64_t id : work)
             for (int f = 0; f < 6; ++f) {
                 const int64_t nb = neighbour_node(m, id, f);
                 if (nb < 0)
@@ -342,7 +396,9 @@ drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *c
     if (!m || !type || !count || !elems || !faces || !subs)
         return drf::fail(DRF_ERR_ARG, "null argument");
     const int64_t E = (int64_t)m->leaves.size();
-    for (int64_t e = 0; e < E; ++e) {
+    // elements are independent: each writes its own six faces
+    parallel_chunks(E, [&](int64_t e0, int64_t e1) {
+    for (int64_t e = e0; e < e1; ++e) {
         const int64_t id = m->leaves[e];
         const Node &nd = m->nodes[id];
         for (int f = 0; f < 6; ++f) {
@@ -396,6 +452,7 @@ drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *c
             }
         }
     }
+    });
     return DRF_OK;
 }
 
