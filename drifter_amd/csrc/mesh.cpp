@@ -208,14 +208,12 @@ bool needs_split(const drf_mesh &m, int64_t id)
 
 // 2:1 balance, pass by pass over the leaves in leaf order, splitting a
 // neighbour more than one level coarser on the axes that need it.  A leaf
-// that existed when the previous pass began cannot force a split in a later
-// pass (its neighbours only became finer), so passes after the first visit
-// only the leaves created during the previous pass, still in leaf order: the
-// same splits in the same order as rescanning every leaf.  Within a pass, a
-// leaf with no such neighbour when the pass begins gets none later in the
-// pass (splits only make neighbours finer), so the leaves are first filtered
-// in parallel against the tree as it stands, and only those flagged are
-// checked, in order, against the tree as the pass changes it.
+// with no such neighbour gets none later (splits only make neighbours finer),
+// so a pass checks, in leaf order, only the leaves that can still need a
+// split: after the first pass, those the previous pass created and those
+// whose check split a neighbour; and of these only the ones that a parallel
+// filter against the tree at the start of the pass flags.  The splits and
+// their order are those of rescanning every leaf in every pass.
 void balance(drf_mesh &m)
 {
     bool changed = true;
@@ -232,7 +230,9 @@ void balance(drf_mesh &m)
         for (size_t i = 0; i < snapshot.size(); ++i)
             if (flag[i])
                 work.push_back(snapshot[i]);
-        for (int64_t id : work)
+        std::vector<int64_t> split_by; // leaves whose check split a neighbour
+        for (int64_t id : work) {
+            bool split = false;
             for (int f = 0; f < 6; ++f) {
                 const int64_t nb = neighbour_node(m, id, f);
                 if (nb < 0)
@@ -244,14 +244,23 @@ void balance(drf_mesh &m)
                 if (need && !m.nodes[nb].nchild) {
                     m.nodes[nb].mask = need;
                     create_children(m, nb);
-                    changed = true;
+                    changed = split = true;
                 }
             }
+            if (split)
+                split_by.push_back(id);
+        }
         if (changed) {
+            // next pass: the leaves this pass created, and the leaves that
+            // split a neighbour (one split narrows a level gap by one only,
+            // so they may still need another), in leaf order
             number_leaves(m);
+            std::vector<char> again(m.nodes.size(), 0);
+            for (int64_t id : split_by)
+                again[id] = 1;
             snapshot.clear();
             for (int64_t id : m.leaves)
-                if (id >= first_new)
+                if (id >= first_new || again[id])
                     snapshot.push_back(id);
         }
     }

@@ -70,3 +70,44 @@ def test_out_of_range_element():
     with pytest.raises(_lib.DrfError) as ei:
         p.element_bounds(8)
     assert ei.value.status == _lib.DRF_ERR_RANGE
+
+
+def test_refine_balance_sequence_large(orc):
+    """Random anisotropic refinement of a 32x32x4 base over three rounds
+    (beyond 4,096 leaves, where balance filters and connectivity run on
+    several threads), with the balance cascades that mixed masks cause."""
+    rng = np.random.default_rng(11)
+    p = mesh.OctreeAdapter(0.0, 2.0, 0.0, 1.0, -1.0, 0.0)
+    o = orc.Mesh(0.0, 2.0, 0.0, 1.0, -1.0, 0.0)
+    p.build_uniform(32, 32, 4)
+    o.build_uniform(32, 32, 4)
+    for _ in range(3):
+        E = p.num_elements()
+        sel = np.unique(rng.integers(0, E, size=E // 6))
+        masks = rng.integers(1, 8, size=sel.size)
+        p.refine(sel, masks)
+        o.refine(sel, masks)
+    assert p.num_elements() > 2 * 4096
+    _compare(p, o)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_deep_local_refinement_cascades(orc, seed):
+    """Refining the leaves near one point again and again opens level gaps of
+    several levels at once, so balance needs several passes and leaves that
+    split a neighbour must be checked again in the next pass."""
+    rng = np.random.default_rng(seed)
+    p = mesh.OctreeAdapter(0.0, 1.0, 0.0, 1.0, -1.0, 0.0)
+    o = orc.Mesh(0.0, 1.0, 0.0, 1.0, -1.0, 0.0)
+    p.build_uniform(32, 32, 4)
+    o.build_uniform(32, 32, 4)
+    c = rng.uniform(0.2, 0.8, 2)
+    for r in range(4):
+        b, _, _ = p.elements()
+        cx, cy = 0.5 * (b[:, 0] + b[:, 1]), 0.5 * (b[:, 2] + b[:, 3])
+        sel = np.nonzero((np.abs(cx - c[0]) < 0.02) & (np.abs(cy - c[1]) < 0.02))[0]
+        masks = rng.integers(1, 8, size=sel.size)
+        p.refine(sel, masks)
+        o.refine(sel, masks)
+    assert p.num_elements() > 4096
+    _compare(p, o)
