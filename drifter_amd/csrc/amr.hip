@@ -42,10 +42,12 @@ __device__ __forceinline__ int64_t parent_source(BoxMode mode, int64_t p,
 // Children of PPB parents per batch.  Transfer: parents idx[p], XYZ, written
 // after the n_keep copies; Children: parent p = element p, the given mask.
 template <int N, BoxMode MODE>
-__global__ void __launch_bounds__(kThreads)
-    children_kernel(int mask, int64_t n_parent, int64_t n_src, const double *__restrict__ U,
-                    int64_t es, int64_t base, const int64_t *__restrict__ idx, NodeSet ns,
-                    double *__restrict__ out, int64_t os)
+__device__ __forceinline__ void children_body(int64_t bid, int64_t nblk, int mask,
+                                              int64_t n_parent, int64_t n_src,
+                                              const double *__restrict__ U, int64_t es,
+                                              int64_t base, const int64_t *__restrict__ idx,
+                                              const NodeSet &ns, double *__restrict__ out,
+                                              int64_t os)
 {
     using S = Shape<N>;
     constexpr int N2 = S::N2, ND = S::ND, PPB = S::PPB;
@@ -75,10 +77,10 @@ __global__ void __launch_bounds__(kThreads)
             }
         }
     };
-    int64_t b = blockIdx.x;
+    int64_t b = bid;
     if (b < nb)
         fetch(b);
-    for (; b < nb; b += gridDim.x) {
+    for (; b < nb; b += nblk) {
         const int64_t p0 = b * PPB;
         const int cnt = (n_parent - p0 < PPB) ? (int)(n_parent - p0) : PPB;
         // sU was last read before two barriers of the previous batch
@@ -88,8 +90,8 @@ __global__ void __launch_bounds__(kThreads)
             if (q < cnt * ND)
                 sU[q] = nxt[v];
         }
-        if (b + gridDim.x < nb)
-            fetch(b + gridDim.x);
+        if (b + nblk < nb)
+            fetch(b + nblk);
         __syncthreads();
         // xi: T1[e][hx][k][j][p] = sum_i Mx_hx[p][i] u[e][k][j][i]
         // nx, ny, nz are 1 or 2: unit u = q / ND splits by shifts and masks
@@ -136,20 +138,31 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+template <int N, BoxMode MODE>
+__global__ void __launch_bounds__(kThreads)
+    children_kernel(int mask, int64_t n_parent, int64_t n_src, const double *__restrict__ U,
+                    int64_t es, int64_t base, const int64_t *__restrict__ idx, NodeSet ns,
+                    double *__restrict__ out, int64_t os)
+{
+    children_body<N, MODE>(blockIdx.x, gridDim.x, mask, n_parent, n_src, U, es, base, idx, ns,
+                           out, os);
+}
+
 // transfer_solution's copies: output o < n_keep is the o-th element that is
 // not a parent, o + #{j : idx[j] - j <= o} (idx strictly ascending, so
 // idx[j] - j is non-decreasing).  One binary search per element.
 template <int N>
-__global__ void __launch_bounds__(kThreads)
-    copy_ranked_kernel(int64_t n_keep, int64_t n_src, const double *__restrict__ U, int64_t es,
-                       int64_t n_parent, const int64_t *__restrict__ idx,
-                       double *__restrict__ out, int64_t os)
+__device__ __forceinline__ void copy_ranked_body(int64_t bid, int64_t nblk, int64_t n_keep,
+                                                 int64_t n_src, const double *__restrict__ U,
+                                                 int64_t es, int64_t n_parent,
+                                                 const int64_t *__restrict__ idx,
+                                                 double *__restrict__ out, int64_t os)
 {
     constexpr int ND = N * N * N;
     __shared__ int64_t sSrc[kCopyChunk];
     const int t = threadIdx.x;
     const int64_t nb = (n_keep + kCopyChunk - 1) / kCopyChunk;
-    for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    for (int64_t b = bid; b < nb; b += nblk) {
         const int64_t o0 = b * kCopyChunk;
         const int cnt = (n_keep - o0 < kCopyChunk) ? (int)(n_keep - o0) : kCopyChunk;
         if (t < cnt) {
@@ -173,6 +186,24 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
     }
+}
+
+// transfer_solution in one launch: the first n_copy_blk workgroups make the
+// copies, the others the parents' children.  The two parts write disjoint
+// outputs and are each too small to fill the GPU alone, so running them side
+// by side instead of one after the other halves the transfer's time.
+template <int N>
+__global__ void __launch_bounds__(kThreads)
+    transfer_kernel(int64_t n_copy_blk, int64_t n_keep, int64_t n_parent, int64_t n_src,
+                    const double *__restrict__ U, int64_t es, const int64_t *__restrict__ idx,
+                    NodeSet ns, double *__restrict__ out, int64_t os)
+{
+    if ((int64_t)blockIdx.x < n_copy_blk)
+        copy_ranked_body<N>(blockIdx.x, n_copy_blk, n_keep, n_src, U, es, n_parent, idx, out, os);
+    else
+        children_body<N, BoxMode::Transfer>((int64_t)blockIdx.x - n_copy_blk,
+                                            (int64_t)gridDim.x - n_copy_blk, 7, n_parent, n_src,
+                                            U, es, n_keep, idx, ns, out, os);
 }
 
 // M[p][i] = L_i(lo + (x_p + 1) (hi - lo) / 2), the reference's arithmetic.
@@ -385,15 +416,14 @@ hipError_t interp_n(BoxMode mode, int mask, int64_t n_out, int64_t n_src, const 
     using S = Shape<N>;
     const dim3 blk(kThreads);
     switch (mode) {
-    case BoxMode::Transfer:
-        if (n_keep > 0)
-            hipLaunchKernelGGL((copy_ranked_kernel<N>), dim3(grid_for<kCopyChunk>(n_keep)), blk, 0,
-                               s, n_keep, n_src, U, es, n_parent, idx, out, os);
-        if (n_parent > 0)
-            hipLaunchKernelGGL((children_kernel<N, BoxMode::Transfer>),
-                               dim3(grid_for<S::PPB>(n_parent)), blk, 0, s, 7, n_parent, n_src, U,
-                               es, n_keep, idx, ns, out, os);
+    case BoxMode::Transfer: {
+        const int64_t gc = n_keep > 0 ? grid_for<kCopyChunk>(n_keep) : 0;
+        const int64_t gp = n_parent > 0 ? grid_for<S::PPB>(n_parent) : 0;
+        if (gc + gp > 0)
+            hipLaunchKernelGGL((transfer_kernel<N>), dim3((unsigned)(gc + gp)), blk, 0, s, gc,
+                               n_keep, n_parent, n_src, U, es, idx, ns, out, os);
         break;
+    }
     case BoxMode::Children: // every source element is a parent
         hipLaunchKernelGGL((children_kernel<N, BoxMode::Children>), dim3(grid_for<S::PPB>(n_src)),
                            blk, 0, s, mask, n_src, n_src, U, es, (int64_t)0, idx, ns, out, os);
