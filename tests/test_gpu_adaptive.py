@@ -2,7 +2,7 @@
 (src/solver/simulation_driver.cpp:108-131) with the CFL time step computed on
 the GPU (compute_max_wave_speed :153-168, AdaptiveTimeController::compute_dt
 src/solver/time_stepper.cpp:261-268) and read by the fused stage kernels.
-It must equal, bit for bit, the host loop the driver runs: wave speed to the
+It must equal, bit for bit, the host loop SimulationDriver::step runs: wave speed to the
 host, compute_dt there, a fused step with that dt; and a captured graph of
 such steps must equal the eager steps over several replays.  One step is
 also checked against the oracle driven the same way."""
