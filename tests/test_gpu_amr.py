@@ -272,3 +272,20 @@ def test_interpolate_boxes_any_boxes_invalid_sources_grid_stride(torch, orc, ord
     assert np.abs(got[ok] - want).max() <= 1e-12 * scale
     whole = ok & np.all(boxes == np.array([-1, 1, -1, 1, -1, 1.0]), axis=1)
     np.testing.assert_array_equal(got[whole], U[src[whole]])
+
+
+def test_transfer_many_copies_few_parents(torch, orc):
+    """transfer_solution's copies and children share one launch: more kept
+    elements than one pass of the copy workgroups (4,096 x 16) beside a few
+    parents, against the oracle (copies bit-exact)."""
+    from drifter_amd import amr
+
+    order, nd, E = 2, 27, 100_000
+    U = np.random.default_rng(9).standard_normal((E, nd))
+    parents = np.array([0, 17, 65_540, 99_999])
+    print(f"transfer E={E} parents={parents.size}", flush=True)
+    got = amr.transfer_solution(order, torch.from_numpy(U).cuda(), parents).cpu().numpy()
+    want = orc.transfer_solution(order, U, parents)
+    keep = E - parents.size
+    np.testing.assert_array_equal(got[:keep], want[:keep])
+    _close(got[keep:], want[keep:])
