@@ -561,12 +561,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // Face traces of a nodal field: out[k][q] = field[elems[k] * es + node(face[k], q)].
+// The node count per edge is a template argument, so the index divisions are
+// multiplications.
+template <int n>
 __global__ void __launch_bounds__(256)
-    gather_faces_kernel(int n, int64_t nelem, int64_t es, int64_t nfaces,
+    gather_faces_kernel(int64_t nelem, int64_t es, int64_t nfaces,
                         const int64_t *__restrict__ elems, const int8_t *__restrict__ faces,
                         const double *__restrict__ field, double *__restrict__ out)
 {
-    const int nf = n * n;
+    constexpr int nf = n * n;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfaces * nf)
         return;
@@ -651,8 +654,23 @@ hipError_t launch_gather_faces(int n, int64_t nelem, int64_t elem_stride, int64_
     const int64_t total = nfaces * n * n;
     if (total <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(gather_faces_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       n, nelem, elem_stride, nfaces, elems, faces, field, out);
+    const dim3 g((unsigned)((total + 255) / 256)), b(256);
+#define DRF_GATHER(NN)                                                                         \
+    hipLaunchKernelGGL(gather_faces_kernel<NN>, g, b, 0, s, nelem, elem_stride, nfaces, elems,  \
+                       faces, field, out);                                                     \
+    break
+    switch (n) {
+    case 2: DRF_GATHER(2);
+    case 3: DRF_GATHER(3);
+    case 4: DRF_GATHER(4);
+    case 5: DRF_GATHER(5);
+    case 6: DRF_GATHER(6);
+    case 7: DRF_GATHER(7);
+    case 8: DRF_GATHER(8);
+    case 9: DRF_GATHER(9);
+    default: return hipErrorInvalidValue;
+    }
+#undef DRF_GATHER
     return hipGetLastError();
 }
 
