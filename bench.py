@@ -1,19 +1,21 @@
 """Benchmark: DG right-hand side throughput on one MI355X.
 
 Workload (one "step" = one evaluation of the full right-hand side):
-uniform 64 x 64 x 8 octree, order N = 4 (125 LGL nodes per element,
-4,096,000 DOF per field), 100 km square, tanh shelf 20 -> 200 m, seeded
-Fourier state; FP64 storage and arithmetic (the parity path), device-resident
-structure-of-arrays state, Coriolis f0 = 1e-4, beta = 0.
+uniform 128 x 128 x 16 octree, order N = 4 (262,144 elements, 125 LGL nodes
+per element, 32,768,000 DOF per field), 100 km square, tanh shelf
+20 -> 200 m, seeded Fourier state; FP64 storage and arithmetic (the parity
+path), device-resident structure-of-arrays state, Coriolis f0 = 1e-4,
+beta = 0.
 
-value = GDOF/s = E * (N+1)^3 * 5 / t_RHS / 1e9 with inputs already in HBM:
-the timed launches cycle over four input/output sets (1.3 GB), so that every
-launch reads its state from HBM, not from the 256 MB last-level cache that
-holds part of a single repeated input/output pair (361 MB); that pair's
-time is reported beside the value.
-Extras: the mixed-precision (FP32 volume) kernel with its relative L2 error
-against the FP64 result, fused / un-fused RK4 steps, and the oracle's FP64
-CPU baseline timed on a bounded sample of elements.
+value = GDOF/s = E * (N+1)^3 * 5 / t_RHS / 1e9 with inputs already in HBM
+(each launch reads 1.44 GB and writes 1.31 GB, far beyond the 256 MB
+last-level cache), timed with HIP events on the launch stream.
+Extras: the mixed-precision (FP32 volume) kernel on the same mesh with its
+error against the FP64 result per RHS and after 1 and 100 RK4 steps at the
+driver's CFL time step, fused / un-fused RK4 steps, the 64 x 64 x 8 mesh,
+the coastline octree, the face, transfer, sigma, adaptation and indicator
+components, and the oracle's FP64 CPU baseline on a bounded sample of
+elements (1 thread and the host's threads).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -80,32 +82,77 @@ def rel_l2_fields(a, b, E, nd):
     return [float(np.linalg.norm(A[f] - B[f]) / np.linalg.norm(B[f])) for f in range(5)]
 
 
-def cpu_baseline(case, target_s=12.0):
-    """Oracle (dense Kronecker mat-vecs, the reference algorithm) on 1 thread
-    over a bounded prefix of the mesh's elements."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads this process may use, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(case, label, target_s=10.0):
+    """The oracle's FP64 right-hand side (dense Kronecker mat-vecs, the
+    reference algorithm) on a bounded prefix of the mesh's elements: once on 1
+    thread (the reference as built: CMake forces OpenMP off) and once with the
+    elements split over the host threads (as the reference's element-loop
+    pragma would, were OpenMP on).  The element-local RHS makes every chunk
+    an independent call; ctypes releases the interpreter lock during each."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    f = np.full(case.h.size, 1e-4)
+    oracle.load()
     nd = case.nd
+    f = np.full(case.h.size, 1e-4)
+
+    def run(e0, e1):
+        oracle.rhs(case.order, case.U[e0 * 5 * nd:e1 * 5 * nd], case.h[e0 * nd:e1 * nd],
+                   f[e0 * nd:e1 * nd])
+
     probe = 256
     t = time.perf_counter()
-    oracle.rhs(case.order, case.U, case.h, f, e0=0, e1=probe)
+    run(0, probe)
     per = (time.perf_counter() - t) / probe
     ns = int(min(case.E, max(probe, target_s / max(per, 1e-9))))
     t = time.perf_counter()
-    oracle.rhs(case.order, case.U, case.h, f, e0=0, e1=ns)
-    dt = time.perf_counter() - t
+    run(0, ns)
+    dt1 = time.perf_counter() - t
+    nt = cpu_threads()
+    nm = int(min(case.E, max(nt * probe, nt * target_s / max(per, 1e-9))))
+    cuts = np.linspace(0, nm, nt + 1).astype(int)
+    with ThreadPoolExecutor(nt) as ex:
+        t = time.perf_counter()
+        list(ex.map(lambda k: run(cuts[k], cuts[k + 1]), range(nt)))
+        dtm = time.perf_counter() - t
+    model = cpu_model()
     return {
-        "value": ns * nd * 5 / dt / 1e9,
+        "value": ns * nd * 5 / dt1 / 1e9,
         "unit": "GDOF/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{ns} of {case.E} elements of the 64x64x8 N=4 mesh, one RHS "
-                  f"({dt:.1f} s), dense Kronecker mat-vecs as the reference computes "
-                  "them, 1 thread (the reference builds with OpenMP off)",
+        "sample": f"{ns} of {case.E} elements of the {label} mesh, one RHS ({dt1:.1f} s), "
+                  "dense Kronecker mat-vecs as the reference computes them, 1 thread (the "
+                  "reference builds with OpenMP off)",
+        "cpu_model": model,
+        "threaded": {
+            "value": nm * nd * 5 / dtm / 1e9,
+            "cores": nt,
+            "sample": f"{nm} elements over {nt} threads ({dtm:.1f} s)",
+        },
     }
 
 
@@ -153,49 +200,123 @@ def copy_probe(steps, warmup):
     }), flush=True)
 
 
-def measure_extras(args, case, s64, U, r64, stream, tune=None):
-    """Mixed-precision RHS (with its error against FP64) and RK4 steps."""
+def measure_mixed_and_steps(args, case, s64, U, r64, stream, label):
+    """On the headline mesh: the mixed-precision (FP32 volume) RHS with its
+    error against the FP64 result, the RK4 step fused / un-fused / mixed, and
+    the mixed state's error against the FP64 state after 1 and 100 RK4 steps
+    at the time step SimulationDriver::step takes from this state (CFL 0.5,
+    dx_min = 2/(N+1)).  The FP64 device path is the reference for these
+    errors: the tests hold it within 1e-12 of the oracle per step."""
     import torch
 
     from drifter_amd import _lib
-    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+    from drifter_amd.solver import (AdaptiveTimeController, CoriolisParameter,
+                                    PrimitiveEquationsSolver)
 
     E, nd = case.E, case.nd
     dof = E * nd
-    # mixed precision (FP32 volume arithmetic and storage)
     s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
     R32 = torch.empty_like(U32)
-    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=8, stream=stream)
-    ms32_warm = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
-                              args.steps, args.warmup, stream)
+    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
+    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
+                         args.steps, args.warmup, stream)
     err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
-    # 8 sets of 164 MB: state from HBM every launch, as for the FP64 value
-    ms32 = time_rotating(s32, U32, R32, 8, args.steps, args.warmup, stream)
-    log(f"mixed RHS: {ms32 * 1e3:.1f} us ({ms32_warm * 1e3:.1f} us on one input/output pair), "
+    del R32
+    log(f"mixed RHS ({label}): {ms32 * 1e3:.1f} us, {dof * 5 / (ms32 * 1e-3) / 1e9:.1f} GDOF/s, "
         f"max rel L2 vs FP64 {max(err32):.2e}")
 
-    # RK4 steps: fused epilogue vs the reference's un-fused sequence
-    dt = 1e-3
+    c = s64.max_wave_speed_dev(U, _lib.LAYOUT_LINES, stream)
+    dt = AdaptiveTimeController(0.5, 1e-6, 3600.0).compute_dt(c, 2.0 / 5)
     Uw = U.clone()
     ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
                                                     True, stream), 10, 2, stream)
-    Uw = U.clone()
+    Uw.copy_(U)
     ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
                                                     False, stream), 10, 2, stream)
     Uw32 = U32.clone()
     ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_LINES,
                                                     True, stream), 10, 2, stream)
-    log(f"RK4 step FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
+    del Uw32
+    log(f"RK4 step ({label}) FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
         f"{ms_rk_m:.3f} ms")
+    steps_err = {}
+    Uw.copy_(U)
+    Ub = U32
+    for k in range(1, 101):
+        s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES, True, stream)
+        s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Ub, _lib.LAYOUT_LINES, True, stream)
+        if k in (1, 100):
+            e = rel_l2_fields(Ub.double().cpu().numpy(), Uw.cpu().numpy(), E, nd)
+            steps_err[k] = [float("%.3e" % x) for x in e]
+    log(f"mixed vs FP64 rel L2 after 1 / 100 RK4 steps at dt {dt:.4g}: {steps_err[1]} / "
+        f"{steps_err[100]}")
+    del Uw, Ub, U32
+    s32.close()
+    return {
+        "autotune_mixed": {"wg_per_cu": tune32[0], "xcd_remap": tune32[1],
+                           "us": round(tune32[2], 2)},
+        "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
+        "mixed_ms": round(ms32, 6),
+        "mixed_vs_fp64_gdof_ratio": None,
+        "mixed_hbm_gb_s": round(case.E * nd * 11 * 4 / (ms32 * 1e-3) / 1e9, 1),
+        "mixed_rhs_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
+        "rk4_dt_from_cfl": dt,
+        "mixed_rel_l2_after_1_rk4_step": steps_err[1],
+        "mixed_rel_l2_after_100_rk4_steps": steps_err[100],
+        "rk4_step_ms_fp64_fused": round(ms_rk_f, 4),
+        "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
+        "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
+        "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
+    }
+
+
+def measure_config2(args, stream):
+    """The 64x64x8, N=4 mesh (32,768 elements, one eighth of the headline
+    mesh): the FP64 and mixed RHS with the state read from HBM (launches
+    cycle over input/output sets, since one 361 MB pair is partly held by the
+    256 MB last-level cache), the driver's adaptive step, and mode splitting
+    (10 baroclinic steps x 20 barotropic substeps)."""
+    import torch
+
+    from drifter_amd import _lib, synth
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    case = synth.config2()
+    E, nd = case.E, case.nd
+    dof = E * nd
+    out = {}
+    s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
+    s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    U = torch.from_numpy(synth.to_lines(case.U, E, 5)).to("cuda")
+    R = torch.empty_like(U)
+    tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
+    ms_pair = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
+                            args.steps, args.warmup, stream)
+    ms = time_rotating(s64, U, R, 4, args.steps, args.warmup, stream)
+    out["c2_fp64_us"] = round(ms * 1e3, 2)
+    out["c2_fp64_one_pair_us"] = round(ms_pair * 1e3, 2)
+    out["c2_fp64_gdof_s"] = round(dof * 5 / (ms * 1e-3) / 1e9, 3)
+    out["c2_fp64_hbm_frac"] = round(s64.rhs_bytes() / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    out["c2_autotune_fp64"] = {"wg_per_cu": tune[0], "xcd_remap": tune[1],
+                               "us": round(tune[2], 2)}
+    s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
+    s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    U32 = U.to(torch.float32)
+    R32 = torch.empty_like(U32)
+    s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=8, stream=stream)
+    ms32 = time_rotating(s32, U32, R32, 8, args.steps, args.warmup, stream)
+    out["c2_mixed_us"] = round(ms32 * 1e3, 2)
+    out["c2_mixed_gdof_s"] = round(dof * 5 / (ms32 * 1e-3) / 1e9, 3)
+    log(f"64x64x8 N=4: FP64 RHS {ms * 1e3:.1f} us from HBM ({ms_pair * 1e3:.1f} us on one pair), "
+        f"mixed {ms32 * 1e3:.1f} us")
+    s32.close()
+    del U32, R32
 
     # the driver's step with the CFL dt (SimulationDriver::step): host loop
     # (wave speed to the host, compute_dt, fused step) vs device-resident
-    # (dt on the device, eager and as one graph of 20 steps); dt_max 1e-5
-    # keeps the stiff N = 4 state stable
-    import time
-
+    # (dt on the device, eager and as one graph of 20 steps)
     lib = _lib.load()
     nstep, dx = 20, 2.0 / 5
     Uw = U.clone()
@@ -203,56 +324,29 @@ def measure_extras(args, case, s64, U, r64, stream, tune=None):
     t0 = time.perf_counter()
     for _ in range(nstep):
         c = s64.max_wave_speed_dev(Uw, _lib.LAYOUT_LINES, stream)
-        dtk = lib.drf_compute_dt(c, dx, 0.5, 1e-8, 1e-5)
+        dtk = lib.drf_compute_dt(c, dx, 0.5, 1e-6, 3600.0)
         s64.rk_step_dev(_lib.RK_RK4, 0.0, dtk, Uw, _lib.LAYOUT_LINES, True, stream)
     torch.cuda.synchronize()
     ms_host = (time.perf_counter() - t0) * 1e3 / nstep
-    Uw = U.clone()
+    Uw.copy_(U)
     dtd = torch.zeros(1, dtype=torch.float64, device=U.device)
     td = torch.zeros(1, dtype=torch.float64, device=U.device)
     ms_dev = time_launches(lambda: s64.rk_step_adaptive_dev(
-        _lib.RK_RK4, Uw, dtd, td, cfl=0.5, dt_min=1e-8, dt_max=1e-5, layout=_lib.LAYOUT_LINES,
+        _lib.RK_RK4, Uw, dtd, td, cfl=0.5, dt_min=1e-6, dt_max=3600.0, layout=_lib.LAYOUT_LINES,
         stream=stream), nstep, 2, stream)
-    g = s64.capture_adaptive_steps(_lib.RK_RK4, Uw, dtd, td, n_steps=nstep, cfl=0.5, dt_min=1e-8,
-                                   dt_max=1e-5, layout=_lib.LAYOUT_LINES)
+    g = s64.capture_adaptive_steps(_lib.RK_RK4, Uw, dtd, td, n_steps=nstep, cfl=0.5, dt_min=1e-6,
+                                   dt_max=3600.0, layout=_lib.LAYOUT_LINES)
     ms_graph = time_launches(lambda: g.launch(stream), 3, 1, stream) / nstep
+    g.close()
     log(f"adaptive RK4 step: host loop {ms_host:.3f} ms, device {ms_dev:.3f} ms, "
         f"graph {ms_graph:.3f} ms")
-
-    # mixed precision vs FP64 after 1 and 100 fused RK4 steps from the same
-    # state (dt = 1e-5: the reference-coordinate horizontal diffusion
-    # H nu_h D^2 is stiff at N = 4; the FP64 GPU path is the reference here,
-    # being within 1e-12 of the oracle per RHS)
-    steps_err = {}
-    Ua, Ub = U.clone(), U32.clone()
-    for k in range(1, 101):
-        s64.rk_step_dev(_lib.RK_RK4, 0.0, 1e-5, Ua, _lib.LAYOUT_LINES, True, stream)
-        s32.rk_step_dev(_lib.RK_RK4, 0.0, 1e-5, Ub, _lib.LAYOUT_LINES, True, stream)
-        if k in (1, 100):
-            e = rel_l2_fields(Ub.double().cpu().numpy(), Ua.cpu().numpy(), E, nd)
-            steps_err[k] = [float("%.3e" % x) for x in e]
-    log(f"mixed vs FP64 rel L2 after 1 / 100 RK4 steps: {steps_err[1]} / {steps_err[100]}")
-    return {
-        "autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1],
-                          "us": round(tune[2], 2) if tune[2] == tune[2] else None}
-        if tune else None,
-        "autotune_mixed": {"wg_per_cu": tune32[0], "xcd_remap": tune32[1],
-                           "us": round(tune32[2], 2)},
-        "mixed_rel_l2_after_1_rk4_step": steps_err[1],
-        "mixed_rel_l2_after_100_rk4_steps": steps_err[100],
-        "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
-        "mixed_ms": round(ms32, 6),
-        "mixed_one_pair_ms": round(ms32_warm, 6),
-        "mixed_rel_l2_vs_fp64_per_field": [float("%.3e" % e) for e in err32],
-        "mixed_hbm_gb_s": round(s32.rhs_bytes() / (ms32 * 1e-3) / 1e9, 1),
-        "rk4_step_ms_fp64_fused": round(ms_rk_f, 4),
-        "rk4_step_ms_fp64_unfused": round(ms_rk_u, 4),
-        "rk4_step_ms_mixed_fused": round(ms_rk_m, 4),
-        "mixed_fused_vs_plain_fp64_rk4_speedup": round(ms_rk_u / ms_rk_m, 3),
-        "adaptive_rk4_step_ms_host_loop": round(ms_host, 4),
-        "adaptive_rk4_step_ms_device": round(ms_dev, 4),
-        "adaptive_rk4_step_ms_graph": round(ms_graph, 4),
-    }
+    out["c2_adaptive_rk4_step_ms_host_loop"] = round(ms_host, 4)
+    out["c2_adaptive_rk4_step_ms_device"] = round(ms_dev, 4)
+    out["c2_adaptive_rk4_step_ms_graph"] = round(ms_graph, 4)
+    del Uw, U, R
+    s64.close()
+    out.update(measure_mode_split(case, stream))
+    return out
 
 
 def measure_coastline(args, stream):
@@ -570,14 +664,14 @@ def main():
         copy_probe(args.steps, args.warmup)
         return
 
-    import numpy as np
     import torch
 
     from drifter_amd import _lib, synth
     from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
 
-    log("building 64x64x8 N=4 case ...")
-    case = synth.config2()
+    label = "128x128x16 N=4"
+    log(f"building {label} case ...")
+    case = synth.config4()
     E, nd = case.E, case.nd
     dof = E * nd
     dev = torch.device("cuda", 0)
@@ -594,29 +688,31 @@ def main():
         tune = (wpc, bool(remap), float("nan"))
         log(f"launch shape fixed: {wpc} workgroups/CU, xcd_remap={bool(remap)}")
     else:
-        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
+        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
         log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
-    # The same input and output every launch: 361 MB, of which the 256 MB
-    # last-level cache keeps a part between launches.  Reported beside the
-    # value, not as it.
-    ms_warm = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
-                            args.steps, args.warmup, stream)
-    r64 = R.cpu().numpy()
-    # The value: launches cycle over NSET input/output sets (1.3 GB), so every
-    # launch reads its state from HBM.  Timed last, so that the last `steps`
-    # dispatches of a kernel trace are the timed ones.
-    nset = 4
-    ms = time_rotating(s64, U, R, nset, args.steps, args.warmup, stream)
+    # One input/output pair: each launch reads 1.44 GB and writes 1.31 GB, so
+    # the 256 MB last-level cache holds under a tenth of it between launches
+    # and the state is read from HBM (at 64x64x8 the timed launches had to
+    # cycle over several sets for that).  HIP events on the launch stream.
+    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
+                       args.steps, args.warmup, stream)
     gdof = dof * 5 / (ms * 1e-3) / 1e9
     bytes_per = s64.rhs_bytes()
     achieved = bytes_per / (ms * 1e-3) / 1e9
-    log(f"FP64 RHS: {ms * 1e3:.1f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s "
-        f"(input from HBM; {ms_warm * 1e3:.1f} us on one input/output pair)")
+    log(f"FP64 RHS ({label}): {ms * 1e3:.1f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s")
 
-    extras = None
+    extras = {"autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1],
+                                "us": round(tune[2], 2) if tune[2] == tune[2] else None}}
     if not args.rhs_only:
-        extras = measure_extras(args, case, s64, U, r64, stream, tune)
-        extras.update(measure_mode_split(case, stream))
+        r64 = R.cpu().numpy()
+        extras.update(measure_mixed_and_steps(args, case, s64, U, r64, stream, label))
+        extras["mixed_vs_fp64_gdof_ratio"] = round(extras["mixed_gdof_s"] / gdof, 3)
+        del r64
+    del U, R
+    s64.close()
+    torch.cuda.empty_cache()
+    if not args.rhs_only:
+        extras.update(measure_config2(args, stream))
         extras.update(measure_coastline(args, stream))
         extras.update(measure_faces(stream))
         extras.update(measure_amr(stream))
@@ -624,14 +720,19 @@ def main():
         extras.update(measure_adapt_cycle(stream))
         extras.update(measure_indicator(stream))
 
-    cpu = None if args.no_cpu else cpu_baseline(case)
+    cpu = None if args.no_cpu else cpu_baseline(case, label)
     if cpu:
-        log(f"CPU baseline: {cpu['value']:.4f} GDOF/s ({cpu['sample']})")
+        log(f"CPU baseline: {cpu['value']:.4f} GDOF/s on 1 thread, "
+            f"{cpu['threaded']['value']:.4f} GDOF/s on {cpu['threaded']['cores']} threads "
+            f"({cpu['cpu_model']})")
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_64x64x8_n4.traffic.json")
+    traffic, tsrc = None, None
+    tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_128x128x16_n4.traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        tj = json.load(open(tfile))
+        traffic = tj.get("bytes_per_launch")
+        tsrc = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and launch shape, "
+                "from profiles/" + os.path.basename(tfile))
 
     line = {
         "metric": "DG RHS GDOF/s on 1 MI355X (FP64 vs mixed @ stated L2 err); % HBM roofline",
@@ -647,7 +748,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "DG RHS, uniform 64x64x8 octree, N=4, tanh shelf, FP64, SoA (xi-planes) in HBM",
+            "workload": "DG RHS, uniform 128x128x16 octree, N=4, tanh shelf, FP64, "
+                        "structure-of-arrays (xi-planes) in HBM",
             "elements": E,
             "order": 4,
             "dof_per_field": dof,
@@ -660,9 +762,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": tsrc,
             "algorithmic_bytes_per_launch": bytes_per,
-            "input": f"{nset} input/output sets in rotation (state read from HBM every launch)",
-            "one_pair_us": round(ms_warm * 1e3, 1),
+            "kernel": "volume_kernel<4, double, double, EPI_RHS>",
         },
         "cpu_baseline": cpu,
         "extras": extras,

@@ -9,7 +9,9 @@ Names, argument meanings and error behaviour follow the reference:
   ModeSplittingSolver           include/physics/mode_splitting.hpp:203-266
   SimulationDriver (step only)  src/solver/simulation_driver.cpp:108-168
 Packed states are numpy float64 vectors [elem][Hu|Hv|eta|HT|HS][ndof]; the
-`*_dev` methods take device buffers (torch tensors or raw pointers).
+`*_dev` methods take contiguous torch device tensors of the context's
+precision (float64, or float32 for the mixed-precision state), checked for
+size, dtype and device before the library is called.
 """
 import ctypes as C
 
@@ -26,6 +28,34 @@ def _ptr(x):
     if hasattr(x, "data_ptr"):
         return C.c_void_p(x.data_ptr())
     return C.c_void_p(int(x))
+
+
+def _host_state(U, n, name="U"):
+    """A C-contiguous float64 copy-free view of a host array of exactly n values
+    (n = 0 before initialize: the library then reports the missing mesh)."""
+    a = np.ascontiguousarray(U, dtype=np.float64)
+    if n and a.size != n:
+        raise ValueError(f"{name} holds {a.size} values, the mesh needs {n}")
+    return a
+
+
+def _dev_array(t, n, dtype, name):
+    """Device pointer of a contiguous torch tensor on the GPU holding at least
+    n values of `dtype` (a torch dtype); anything else raises ValueError, so
+    that no kernel reads or writes past a caller's buffer."""
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the GPU")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor, got one on {t.device}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{name} holds {t.numel()} values, needs {n}")
+    return C.c_void_p(t.data_ptr())
 
 
 def _sptr(s):
@@ -77,6 +107,23 @@ class PrimitiveEquationsSolver:
     def ctx(self):
         return self._ctx
 
+    def state_len(self):
+        """Values in one packed / SoA / line-layout state: E * 5 * ndof."""
+        return self.num_elements * 5 * self.ndof
+
+    def _state_dtype(self):
+        import torch
+
+        return torch.float64 if self.precision == _lib.PREC_FP64 else torch.float32
+
+    def _dev_state(self, t, name="U"):
+        return _dev_array(t, self.state_len(), self._state_dtype(), name)
+
+    def _dev_scalar(self, t, name):
+        import torch
+
+        return None if t is None else _dev_array(t, 1, torch.float64, name)
+
     def set_physics(self, **kw):
         p = _lib.DrfPhysics()
         self._l.drf_default_physics(C.byref(p))
@@ -98,7 +145,7 @@ class PrimitiveEquationsSolver:
 
     def compute_rhs(self, U, t=0.0):
         """SimulationDriver::compute_rhs on the packed layout (host arrays)."""
-        U = np.ascontiguousarray(U, dtype=np.float64)
+        U = _host_state(U, self.state_len())
         out = np.empty_like(U)
         check(self._l.drf_rhs(self._ctx, t, dptr(U), dptr(out)))
         return out
@@ -122,28 +169,29 @@ class PrimitiveEquationsSolver:
         """Time every launch shape on (U, R), keep the fastest; returns
         (wg_per_cu, xcd_remap, microseconds per launch)."""
         us = C.c_float()
-        check(self._l.drf_autotune(self._ctx, _ptr(U), _ptr(R), layout, reps, _sptr(stream),
-                                   C.byref(us)))
+        check(self._l.drf_autotune(self._ctx, self._dev_state(U), self._dev_state(R, "dUdt"),
+                                   layout, reps, _sptr(stream), C.byref(us)))
         w, r = self.get_launch()
         return w, r, us.value
 
     def compute_rhs_dev(self, U, dUdt, layout=_lib.LAYOUT_SOA, stream=None, t=0.0):
-        check(self._l.drf_rhs_dev(self._ctx, t, _ptr(U), _ptr(dUdt), layout, _sptr(stream)))
+        check(self._l.drf_rhs_dev(self._ctx, t, self._dev_state(U), self._dev_state(dUdt, "dUdt"),
+                                  layout, _sptr(stream)))
 
     def rk_step(self, scheme, t, dt, U):
-        U = np.array(U, dtype=np.float64, copy=True)
+        U = np.array(_host_state(U, self.state_len()), copy=True)
         check(self._l.drf_rk_step(self._ctx, scheme, t, dt, dptr(U)))
         return U
 
     def rk_step_dev(self, scheme, t, dt, U, layout=_lib.LAYOUT_SOA, fused=True, stream=None):
-        check(self._l.drf_rk_step_dev(self._ctx, scheme, t, dt, _ptr(U), layout, int(fused),
-                                      _sptr(stream)))
+        check(self._l.drf_rk_step_dev(self._ctx, scheme, t, dt, self._dev_state(U), layout,
+                                      int(fused), _sptr(stream)))
 
     def capture_rk_steps(self, scheme, dt, U, layout=_lib.LAYOUT_SOA, fused=True, n_steps=1):
         """StepGraph of n_steps drf_rk_step_dev calls on U (in place)."""
         h = C.c_void_p()
-        check(self._l.drf_graph_rk(self._ctx, scheme, dt, _ptr(U), layout, int(fused), n_steps,
-                                   C.byref(h)))
+        check(self._l.drf_graph_rk(self._ctx, scheme, dt, self._dev_state(U), layout, int(fused),
+                                   n_steps, C.byref(h)))
         return StepGraph(h)
 
     def rk_step_adaptive_dev(self, scheme, U, dt_dev, time_dev=None, cfl=0.5, dt_min=1e-6,
@@ -153,7 +201,9 @@ class PrimitiveEquationsSolver:
         step.  dx_min defaults to the driver's 2 / (N + 1)."""
         dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
         check(self._l.drf_rk_step_adaptive_dev(self._ctx, scheme, cfl, dx, dt_min, dt_max,
-                                               _ptr(U), layout, _ptr(dt_dev), _ptr(time_dev),
+                                               self._dev_state(U), layout,
+                                               self._dev_scalar(dt_dev, "dt_dev"),
+                                               self._dev_scalar(time_dev, "time_dev"),
                                                _sptr(stream)))
 
     def capture_adaptive_steps(self, scheme, U, dt_dev, time_dev=None, n_steps=1, cfl=0.5,
@@ -161,26 +211,34 @@ class PrimitiveEquationsSolver:
         """StepGraph of n_steps rk_step_adaptive_dev calls on U (in place)."""
         dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
         h = C.c_void_p()
-        check(self._l.drf_graph_rk_adaptive(self._ctx, scheme, cfl, dx, dt_min, dt_max, _ptr(U),
-                                            layout, n_steps, _ptr(dt_dev), _ptr(time_dev),
-                                            C.byref(h)))
+        check(self._l.drf_graph_rk_adaptive(self._ctx, scheme, cfl, dx, dt_min, dt_max,
+                                            self._dev_state(U), layout, n_steps,
+                                            self._dev_scalar(dt_dev, "dt_dev"),
+                                            self._dev_scalar(time_dev, "time_dev"), C.byref(h)))
         return StepGraph(h)
 
     def max_wave_speed(self, U):
-        U = np.ascontiguousarray(U, dtype=np.float64)
+        U = _host_state(U, self.state_len())
         c = C.c_double()
         check(self._l.drf_max_wave_speed(self._ctx, dptr(U), C.byref(c)))
         return c.value
 
     def max_wave_speed_dev(self, U, layout=_lib.LAYOUT_SOA, stream=None):
         c = C.c_double()
-        check(self._l.drf_max_wave_speed_dev(self._ctx, _ptr(U), layout, C.byref(c), _sptr(stream)))
+        check(self._l.drf_max_wave_speed_dev(self._ctx, self._dev_state(U), layout, C.byref(c),
+                                             _sptr(stream)))
         return c.value
 
     def max_wave_speed_derived_dev(self, H, u, v, elem_stride, stream=None):
+        import torch
+
+        if elem_stride < self.ndof:
+            raise ValueError(f"elem_stride {elem_stride} is below ndof {self.ndof}")
+        n = (self.num_elements - 1) * elem_stride + self.ndof
         c = C.c_double()
-        check(self._l.drf_max_wave_speed_derived_dev(self._ctx, _ptr(H), _ptr(u), _ptr(v),
-                                                     elem_stride, C.byref(c), _sptr(stream)))
+        check(self._l.drf_max_wave_speed_derived_dev(
+            self._ctx, _dev_array(H, n, torch.float64, "H"), _dev_array(u, n, torch.float64, "u"),
+            _dev_array(v, n, torch.float64, "v"), elem_stride, C.byref(c), _sptr(stream)))
         return c.value
 
     def rhs_bytes(self):
@@ -281,8 +339,16 @@ class ModeSplittingSolver:
                    y_positions=None):
         self._s.initialize(num_elements, bathymetry, dh_dx, dh_dy, coriolis, y_positions)
 
+    def _full_len(self):
+        return self._s.num_elements * 12 * self._s.ndof
+
+    def _dev_full(self, t):
+        import torch
+
+        return _dev_array(t, self._full_len(), torch.float64, "full_state")
+
     def step(self, dt_3d, full_state):
-        full = np.array(full_state, dtype=np.float64, copy=True)
+        full = np.array(_host_state(full_state, self._full_len(), "full_state"), copy=True)
         check(self._s._l.drf_mode_split_step(self._s.ctx, dt_3d, self.subcycles, dptr(full)))
         return full
 
@@ -290,12 +356,12 @@ class ModeSplittingSolver:
         """StepGraph of n_steps step_dev calls on the device full state."""
         h = C.c_void_p()
         check(self._s._l.drf_graph_mode_split(self._s.ctx, dt_3d, self.subcycles,
-                                              _ptr(full_state), n_steps, C.byref(h)))
+                                              self._dev_full(full_state), n_steps, C.byref(h)))
         return StepGraph(h)
 
     def step_dev(self, dt_3d, full_state, stream=None):
         check(self._s._l.drf_mode_split_step_dev(self._s.ctx, dt_3d, self.subcycles,
-                                                 _ptr(full_state), _sptr(stream)))
+                                                 self._dev_full(full_state), _sptr(stream)))
 
 
 class SimulationConfig:

@@ -18,6 +18,8 @@ definitions are this project's own:
   S = 32.5 + 2.5 F; H = eta + h and the conserved state is H * (u, v, T, S).
 """
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -106,9 +108,10 @@ def node_coords(bounds, nodes):
     return np.ascontiguousarray(x), np.ascontiguousarray(y), np.ascontiguousarray(s)
 
 
-def fourier_field(q, x, y, s, Lx, Ly, seed_base=20261202):
+def fourier_modes(q, seed_base=20261202):
+    """The 8 modes (kx, ky, kz, a, phi) of field q, drawn from splitmix64."""
     rng = SplitMix64(seed_base + q)
-    F = np.zeros_like(x)
+    modes = []
     for m in range(1, 9):
         kx = rng.next() % 4
         ky = rng.next() % 4
@@ -117,7 +120,20 @@ def fourier_field(q, x, y, s, Lx, Ly, seed_base=20261202):
             kz = 0
         a = (2.0 * rng.uniform() - 1.0) / m
         phi = 2.0 * math.pi * rng.uniform()
+        modes.append((kx, ky, kz, a, phi))
+    return modes
+
+
+def fourier_sum(q, x, y, s, Lx, Ly, seed_base=20261202):
+    """Un-normalised sum of field q's 8 modes at the points (x, y, s)."""
+    F = np.zeros_like(x)
+    for kx, ky, kz, a, phi in fourier_modes(q, seed_base):
         F += a * np.cos(2.0 * math.pi * (kx * x / Lx + ky * y / Ly + kz * s) + phi)
+    return F
+
+
+def fourier_field(q, x, y, s, Lx, Ly, seed_base=20261202):
+    F = fourier_sum(q, x, y, s, Lx, Ly, seed_base)
     return F / np.max(np.abs(F))
 
 
@@ -171,26 +187,61 @@ def make_case(nx, ny, nz, order, L=10000.0, bathy="seamount", seed_base=20261202
     return make_case_on(bounds, order, L, bathy, seed_base)
 
 
-def make_case_on(bounds, order, L, bathy="seamount", seed_base=20261202):
-    """Seeded state on any element list (bounds (E, 6), domain [0, L]^2 x [-1, 0])."""
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def make_case_on(bounds, order, L, bathy="seamount", seed_base=20261202, threads=None):
+    """Seeded state on any element list (bounds (E, 6), domain [0, L]^2 x [-1, 0]).
+
+    Every value is a function of its own node, so the elements are evaluated
+    in chunks on a thread pool (numpy releases the interpreter lock inside its
+    loops); the fields are normalised by their maxima over the whole mesh in a
+    second pass.  The result does not depend on the chunking."""
     nodes, _, _ = lgl_nodes(order)
-    x, y, s = node_coords(bounds, nodes)
-    h = bathymetry(bathy, x, y, L, L)
-    u = 0.1 * fourier_field(0, x, y, s, L, L, seed_base)
-    v = 0.1 * fourier_field(1, x, y, s, L, L, seed_base)
-    eta = 0.05 * fourier_field(2, x, y, s, L, L, seed_base)
-    T = 15.0 + 5.0 * fourier_field(3, x, y, s, L, L, seed_base)
-    S = 32.5 + 2.5 * fourier_field(4, x, y, s, L, L, seed_base)
-    H = eta + h
-    E, nd = x.shape
+    bounds = np.ascontiguousarray(bounds, dtype=np.float64)
+    E = bounds.shape[0]
+    nd = (order + 1) ** 3
+    h = np.empty((E, nd))
+    y = np.empty((E, nd))
+    F = np.empty((5, E, nd))
     U = np.empty((E, 5, nd))
-    U[:, 0] = H * u
-    U[:, 1] = H * v
-    U[:, 2] = eta
-    U[:, 3] = H * T
-    U[:, 4] = H * S
-    return Case(order, np.ascontiguousarray(bounds), np.ascontiguousarray(h.reshape(-1)),
-                np.ascontiguousarray(y.reshape(-1)), np.ascontiguousarray(U.reshape(-1)), L, L)
+    chunk = 4096
+    starts = list(range(0, E, chunk))
+    fmax = np.zeros((len(starts), 5))
+
+    def pass1(c):
+        e0 = starts[c]
+        e1 = min(E, e0 + chunk)
+        xc, yc, sc = node_coords(bounds[e0:e1], nodes)
+        h[e0:e1] = bathymetry(bathy, xc, yc, L, L)
+        y[e0:e1] = yc
+        for q in range(5):
+            F[q, e0:e1] = fourier_sum(q, xc, yc, sc, L, L, seed_base)
+            fmax[c, q] = np.max(np.abs(F[q, e0:e1]))
+
+    def pass2(c, M):
+        e0 = starts[c]
+        e1 = min(E, e0 + chunk)
+        u = 0.1 * (F[0, e0:e1] / M[0])
+        v = 0.1 * (F[1, e0:e1] / M[1])
+        eta = 0.05 * (F[2, e0:e1] / M[2])
+        T = 15.0 + 5.0 * (F[3, e0:e1] / M[3])
+        S = 32.5 + 2.5 * (F[4, e0:e1] / M[4])
+        H = eta + h[e0:e1]
+        U[e0:e1, 0] = H * u
+        U[e0:e1, 1] = H * v
+        U[e0:e1, 2] = eta
+        U[e0:e1, 3] = H * T
+        U[e0:e1, 4] = H * S
+
+    nt = threads or _threads()
+    with ThreadPoolExecutor(nt) as ex:
+        list(ex.map(pass1, range(len(starts))))
+        M = fmax.max(axis=0)
+        list(ex.map(lambda c: pass2(c, M), range(len(starts))))
+    del F
+    return Case(order, bounds, h.reshape(-1), y.reshape(-1), U.reshape(-1), L, L)
 
 
 # Named workloads
@@ -202,6 +253,12 @@ def config1():
 def config2():
     """64x64x8 elements, N=4, 100 km square, tanh shelf 20 -> 200 m."""
     return make_case(64, 64, 8, 4, 100000.0, "shelf")
+
+
+def config4():
+    """128x128x16 elements, N=4 (262,144 elements, 32,768,000 DOF per field),
+    100 km square, the tanh shelf 20 -> 200 m of the 64x64x8 case."""
+    return make_case(128, 128, 16, 4, 100000.0, "shelf")
 
 
 def config3(band=7000.0):

@@ -1,0 +1,87 @@
+"""Accuracy of the mixed-precision path (FP32 volume arithmetic and storage,
+FP64 column integrals of the free-surface equation) against the FP64 oracle
+(PrimitiveEquationsSolver::compute_rhs, src/physics/primitive_equations.cpp:428-442;
+RK4Classic, src/solver/time_stepper.cpp:86-119).
+
+Bars, per field, relative L2: right-hand side <= 2e-5 (FP32 rounding of the
+state, the operators and about 2(N+1) accumulated products per derivative);
+state after 1 and 100 RK4 steps <= 1e-5.  Every value must be finite.
+"""
+import numpy as np
+import pytest
+
+from drifter_amd import _lib, synth
+from drifter_amd.solver import AdaptiveTimeController, CoriolisParameter, PrimitiveEquationsSolver
+from sampling import lines_rows, oracle_chunks, rel_l2_fields, rows, sample_elements
+
+pytestmark = pytest.mark.gpu
+TOL_RHS = 2e-5
+TOL_STATE = 1e-5
+
+
+def _solvers(case):
+    m = PrimitiveEquationsSolver(case.order, _lib.PREC_MIXED)
+    m.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    return m
+
+
+def test_mixed_config1_rhs_and_steps(orc):
+    """8x8x4, N=3: every element; 1 and 100 RK4 steps at the driver's CFL dt."""
+    import torch
+
+    case = synth.config1()
+    E, n, nd = case.E, case.n, case.nd
+    m = _solvers(case)
+    U32 = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda().float()
+    R32 = torch.empty_like(U32)
+    m.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES)
+    torch.cuda.synchronize()
+    f = np.full(case.h.size, 1e-4)
+    ref = orc.rhs(case.order, case.U, case.h, f)
+    got = synth.from_lines(R32.double().cpu().numpy(), E, n)
+    assert np.all(np.isfinite(got))
+    errs = rel_l2_fields(got, ref, nd)
+    print(f"[mixed] 8x8x4 N=3 RHS rel L2 {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) <= TOL_RHS, errs
+    H, u, v, _, _ = orc.derived(case.order, case.U, case.h)
+    dt = AdaptiveTimeController(0.5, 1e-6, 3600.0).compute_dt(
+        orc.max_wave_speed(case.order, H, u, v), 2.0 / n)
+    Uref = case.U.copy()
+    done = 0
+    for k in (1, 100):
+        for _ in range(k - done):
+            m.rk_step_dev(_lib.RK_RK4, 0.0, dt, U32, _lib.LAYOUT_LINES, fused=True)
+            Uref = orc.rk_step(orc.RK4, case.order, 0.0, dt, Uref, case.h, f)
+        done = k
+        torch.cuda.synchronize()
+        got = synth.from_lines(U32.double().cpu().numpy(), E, n)
+        assert np.all(np.isfinite(got))
+        errs = rel_l2_fields(got, Uref, nd)
+        print(f"[mixed] after {k} RK4 steps (dt {dt:.4g}) rel L2 {['%.2e' % e for e in errs]}",
+              flush=True)
+        assert max(errs) <= TOL_STATE, (k, errs)
+    m.close()
+
+
+def test_mixed_coastline_rhs_sampled(orc):
+    """The 39,280-element coastline octree, N=3, checked on 400 sampled elements."""
+    import torch
+
+    case, _ = synth.config3()
+    E, n, nd = case.E, case.n, case.nd
+    m = _solvers(case)
+    U32 = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda().float()
+    R32 = torch.empty_like(U32)
+    m.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES)
+    torch.cuda.synchronize()
+    idx = sample_elements(E, 400, seed=3)
+    Usub = rows(case.U, idx, 5 * nd)
+    hsub = rows(case.h, idx, nd)
+    ref = oracle_chunks(lambda u, h: orc.rhs(case.order, u, h, np.full(h.size, 1e-4)), Usub,
+                        hsub, nd)
+    got = lines_rows(R32, idx, E, n)
+    assert np.all(np.isfinite(got))
+    errs = rel_l2_fields(got, ref, nd)
+    print(f"[mixed] coastline E={E} RHS rel L2 {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) <= TOL_RHS, errs
+    m.close()

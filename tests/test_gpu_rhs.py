@@ -141,3 +141,42 @@ def test_launch_shape_does_not_change_results(prec):
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
     s.close()
+
+
+def test_wrappers_reject_wrong_buffers():
+    """Host and device buffers are checked before the library is called:
+    a short, mistyped or host-resident state raises ValueError."""
+    import torch
+
+    case = synth.config1()
+    s = PrimitiveEquationsSolver(case.order)
+    s.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    with pytest.raises(ValueError):
+        s.compute_rhs(case.U[:-1])
+    with pytest.raises(ValueError):
+        s.rk_step(_lib.RK_RK4, 0.0, 1e-4, case.U[:-5])
+    with pytest.raises(ValueError):
+        s.max_wave_speed(case.U[:10])
+    U = torch.from_numpy(case.U).cuda()
+    R = torch.empty_like(U)
+    with pytest.raises(ValueError):
+        s.compute_rhs_dev(U[:-1], R)
+    with pytest.raises(ValueError):
+        s.compute_rhs_dev(U, R[:100])
+    with pytest.raises(ValueError):
+        s.compute_rhs_dev(U.float(), R)
+    with pytest.raises(ValueError):
+        s.compute_rhs_dev(U.cpu(), R)
+    with pytest.raises(ValueError):
+        s.compute_rhs_dev(U.view(5, -1).t(), R)
+    with pytest.raises(ValueError):
+        s.rk_step_dev(_lib.RK_RK4, 0.0, 1e-4, U[:-1])
+    dt = torch.zeros(1, dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        s.rk_step_adaptive_dev(_lib.RK_RK4, U, dt)
+    s.close()
+    m = PrimitiveEquationsSolver(case.order, _lib.PREC_MIXED)
+    m.initialize(case.E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    with pytest.raises(ValueError):  # the mixed state is float32
+        m.compute_rhs_dev(U, R, _lib.LAYOUT_PACKED)
+    m.close()
