@@ -15,15 +15,23 @@
  * the bathymetry and Coriolis parameter of the 2D state are h(i), f(i) for
  * i < n_h (:31, :204-205, :515).
  *
- * Pinned by (tests/test_oracle_modesplit.py): the reference's invariant tests
- * (tests/unit/test_mode_splitting.cpp:52-213, 281-400: depth average of a
- * constant, linearity, deviation, quiescent barotropic RHS = 0, sum of eta
- * conserved by a step in the absence of transport gradients, update_derived
- * values, Coriolis sign), a pure-Python restatement of the predictor-corrector
- * on n = 2 (small enough to be written term by term from :293-332), and
- * closed forms: with zero transport and flat eta the step leaves eta fixed and
- * rescales HT exactly.  Full-step values at N >= 3 beyond these are
- * "parity unpinned" (no golden vectors exist in the reference).
+ * Pinned by (tests/test_oracle_modesplit.py, tests/test_oracle_modesplit_pins.py):
+ * - the reference's component tests (tests/unit/test_mode_splitting.cpp:52-186:
+ *   depth average of a constant, its linearity, depth integral, deviation and
+ *   its zero average, quiescent barotropic RHS, divergence of a constant
+ *   transport) on the element-level entry points below, with closed forms of
+ *   D_x2d, D_y2d, the pressure gradient and the Coriolis signs on linear fields;
+ * - closed forms through the whole step: the gravity term with and without
+ *   Coriolis (Heun's map w -> P w + dt F (1 + z/2)), the free-surface
+ *   divergence with Coriolis, eta_avg through update_baroclinic's H, the
+ *   omega profile and the column its Hinv comes from, the quiescent column,
+ *   uniform transport rotated by Coriolis with g = 0, the eta total of a
+ *   resting flat surface (:189-213), Hu not refreshed by recombine;
+ * - planted mistakes: copies of this file compiled with swapped D_x2d / D_y2d,
+ *   flipped signs, the wrong Hinv column, the final eta in place of eta_avg or
+ *   a one-stage substep each fail at least one of those closed forms.
+ * What none of these fixes: the order in which Eigen sums the dense 2D
+ * mat-vecs (a few ulp).
  */
 #include "oracle.h"
 
@@ -103,6 +111,26 @@ static void depth_integrate(const ms_ops *o, const double *u3, const double *H, 
     }
 }
 
+/* compute_deviation :164-175 */
+static void deviation(const ms_ops *o, const double *u3, const double *ub, double *up)
+{
+    for (int ih = 0; ih < o->nh; ++ih)
+        for (int k = 0; k < o->n; ++k) {
+            const int idx = ih * o->n + k;
+            up[idx] = u3[idx] - ub[ih];
+        }
+}
+
+/* compute_2d_divergence :129-134 */
+static void divergence_2d(const ms_ops *o, const double *HU, const double *HV, double *div)
+{
+    double a[NHMAX], b[NHMAX];
+    mvh(o, o->Dx, HU, a);
+    mvh(o, o->Dy, HV, b);
+    for (int i = 0; i < o->nh; ++i)
+        div[i] = a[i] + b[i];
+}
+
 typedef struct {
     double eta[NHMAX], HU[NHMAX], HV[NHMAX], H[NHMAX], U[NHMAX], V[NHMAX];
 } baro;
@@ -164,12 +192,8 @@ int orc_mode_split_step(int order, int64_t E, double dt3, int nsub, double *full
         depth_integrate(&o, u, H, b.HU);
         depth_integrate(&o, v, H, b.HV);
         memcpy(b.H, H, sizeof(double) * nh);
-        for (int ih = 0; ih < nh; ++ih)
-            for (int k = 0; k < n; ++k) {
-                const int idx = ih * n + k;
-                up[idx] = u[idx] - b.U[ih];
-                vp[idx] = v[idx] - b.V[ih];
-            }
+        deviation(&o, u, b.U, up);
+        deviation(&o, v, b.V, vp);
 
         /* ---- subcycle :319-332 (predictor-corrector :293-317) ---- */
         double eavg[NHMAX];
@@ -229,14 +253,12 @@ int orc_mode_split_step(int order, int64_t E, double dt3, int nsub, double *full
             HT[idx] = H[idx] * T[idx];
             HS[idx] = H[idx] * S[idx];
         }
-        double HUb[NHMAX], HVb[NHMAX], a[NHMAX], bb[NHMAX];
+        double HUb[NHMAX], HVb[NHMAX], dv[NHMAX];
         depth_integrate(&o, u, H, HUb);
         depth_integrate(&o, v, H, HVb);
-        /* compute_2d_divergence :129-134 */
-        mvh(&o, o.Dx, HUb, a);
-        mvh(&o, o.Dy, HVb, bb);
+        divergence_2d(&o, HUb, HVb, dv);
         for (int ih = 0; ih < nh; ++ih) {
-            const double div = a[ih] + bb[ih];
+            const double div = dv[ih];
             const double Hc = H[ih * n];
             const double Hinv = (Hc > 1e-10) ? 1.0 / Hc : 0.0;
             for (int k = 0; k < n; ++k) {
@@ -260,5 +282,60 @@ int orc_mode_split_step(int order, int64_t E, double dt3, int nsub, double *full
             for (int k = 0; k < n; ++k)
                 eta[ih * n + k] = b.eta[ih];
     }
+    return 0;
+}
+
+/* ---- the element-level pieces on their own, for the reference's
+ * component tests (tests/unit/test_mode_splitting.cpp:52-186) ---- */
+
+int orc_ms_depth_average(int order, const double *u3, double *ub)
+{
+    ms_ops o;
+    if (ms_init(&o, order))
+        return -1;
+    depth_average(&o, u3, ub);
+    return 0;
+}
+
+int orc_ms_depth_integrate(int order, const double *u3, const double *H, double *hub)
+{
+    ms_ops o;
+    if (ms_init(&o, order))
+        return -1;
+    depth_integrate(&o, u3, H, hub);
+    return 0;
+}
+
+int orc_ms_deviation(int order, const double *u3, const double *ub, double *up)
+{
+    ms_ops o;
+    if (ms_init(&o, order))
+        return -1;
+    deviation(&o, u3, ub, up);
+    return 0;
+}
+
+int orc_ms_divergence_2d(int order, const double *HU, const double *HV, double *div)
+{
+    ms_ops o;
+    if (ms_init(&o, order))
+        return -1;
+    divergence_2d(&o, HU, HV, div);
+    return 0;
+}
+
+int orc_ms_barotropic_rhs(int order, const double *eta, const double *HU, const double *HV,
+                          const double *H, const double *f, double g, double *de, double *dU,
+                          double *dV)
+{
+    ms_ops o;
+    if (ms_init(&o, order))
+        return -1;
+    baro s;
+    memcpy(s.eta, eta, sizeof(double) * o.nh);
+    memcpy(s.HU, HU, sizeof(double) * o.nh);
+    memcpy(s.HV, HV, sizeof(double) * o.nh);
+    memcpy(s.H, H, sizeof(double) * o.nh);
+    baro_rhs(&o, &s, f, g, de, dU, dV);
     return 0;
 }

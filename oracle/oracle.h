@@ -84,6 +84,16 @@ double orc_compute_dt(double max_wave_speed, double dx_min, double cfl, double d
 /* ---- A11-A12: ModeSplittingSolver::step on the full-state layout ---- */
 int orc_mode_split_step(int order, int64_t n_elem, double dt_3d, int n_sub, double *full,
                         const double *h, const double *f, const orc_physics *phys);
+/* element-level pieces (mode_splitting.cpp:129-221): u3, up have n^3
+ * values in the reference's column layout idx = ih * n + k; ub, hub, HU, HV,
+ * eta, H, f, div and the tendencies have n^2 */
+int orc_ms_depth_average(int order, const double *u3, double *ub);
+int orc_ms_depth_integrate(int order, const double *u3, const double *H, double *hub);
+int orc_ms_deviation(int order, const double *u3, const double *ub, double *up);
+int orc_ms_divergence_2d(int order, const double *HU, const double *HV, double *div);
+int orc_ms_barotropic_rhs(int order, const double *eta, const double *HU, const double *HV,
+                          const double *H, const double *f, double g, double *de, double *dU,
+                          double *dV);
 
 /* ---- A17: octree and face connectivity (src/mesh/octree_adapter.cpp) ---- */
 /* FaceConnectionType order, include/dg/face_connection.hpp:15-46 */

@@ -53,6 +53,11 @@ def load():
         "orc_compute_dt": (C.c_double, [C.c_double] * 5),
         "orc_mode_split_step": (C.c_int, [C.c_int, C.c_int64, C.c_double, C.c_int, _PD, _PD, _PD,
                                           C.POINTER(OrcPhysics)]),
+        "orc_ms_depth_average": (C.c_int, [C.c_int, _PD, _PD]),
+        "orc_ms_depth_integrate": (C.c_int, [C.c_int, _PD, _PD, _PD]),
+        "orc_ms_deviation": (C.c_int, [C.c_int, _PD, _PD, _PD]),
+        "orc_ms_divergence_2d": (C.c_int, [C.c_int, _PD, _PD, _PD]),
+        "orc_ms_barotropic_rhs": (C.c_int, [C.c_int] + [_PD] * 5 + [C.c_double] + [_PD] * 3),
         "orc_mesh_create": (C.c_void_p, [C.c_double] * 6),
         "orc_mesh_destroy": (None, [C.c_void_p]),
         "orc_mesh_build_uniform": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
