@@ -72,6 +72,15 @@ SIGNATURES = {
     "drf_mesh_elements": (_I, [_P, _P, _P, _P]),
     "drf_mesh_element_bounds": (_I, [_P, _L, _PD]),
     "drf_mesh_face_connections": (_I, [_P, _P, _P, _P, _P, _P]),
+    "drf_mesh_face_connections_2to1": (_I, [_P, _P, _P, _P, _P, _P]),
+    "drf_face_plan_create": (_I, [_P, _L, _P, _P, _P, _P, _P, _P, _I, _PD, C.POINTER(_P)]),
+    "drf_face_plan_destroy": (None, [_P]),
+    "drf_face_plan_info": (_I, [_P, C.POINTER(_I), C.POINTER(_L), C.POINTER(_L)]),
+    "drf_face_plan_groups": (_I, [_P, _P]),
+    "drf_face_plan_lists": (_I, [_P, _P, _P]),
+    "drf_face_term_dev": (_I, [_P, _P, _P, _I, _P]),
+    "drf_face_group_fluxes_dev": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+    "drf_set_face_plan": (_I, [_P, _P]),
     "drf_flux_points": (_I, [_I, _L, _P, _P, _P, _PD, _P, _P]),
     "drf_face_map": (_I, [_I, _I, _P]),
     "drf_gather_face_traces": (_I, [_I, _L, _L, _L, _P, _P, _P, _P, _P]),

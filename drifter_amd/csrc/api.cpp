@@ -184,7 +184,11 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
 
 drf_status rhs_into(drf_ctx *c, const void *U, void *R, int layout, hipStream_t s)
 {
-    return volume(c, U, layout, EPI_RHS, 0, R, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0, s);
+    drf_status st =
+        volume(c, U, layout, EPI_RHS, 0, R, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0, s);
+    if (st || !c->faces)
+        return st;
+    return face_term(c->faces, U, R, layout, s);
 }
 
 // Un-fused stepping: the reference's sequence of RHS evaluations and vector
@@ -313,6 +317,9 @@ drf_status rk_unfused(drf_ctx *c, int scheme, double dt, double *U, int layout, 
 drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipStream_t s,
                     const double *dtp = nullptr)
 {
+    DRF_REQUIRE(!c->faces, DRF_ERR_UNSUPPORTED,
+                "fused Runge-Kutta stages carry no face term: step a context with a face plan "
+                "un-fused");
     if (scheme == DRF_RK_DP45) {
         if (dtp)
             return fail(DRF_ERR_UNSUPPORTED, "device time steps need a fused scheme (not DP45)");
@@ -589,6 +596,7 @@ void drf_destroy(drf_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    (void)drf_set_face_plan(c, nullptr);
     (void)hipFree(c->d_h);
     (void)hipFree(c->d_f);
     (void)hipFree(c->d_h32);
@@ -618,6 +626,7 @@ drf_status drf_initialize(drf_ctx *c, int64_t E, const double *h, const double *
     DRF_REQUIRE(h, DRF_ERR_ARG, "bathymetry is required");
     DRF_REQUIRE(y || beta == 0.0, DRF_ERR_ARG, "y_positions are required when beta != 0");
     DRF_HIP_CHECK(hipSetDevice(c->device));
+    (void)drf_set_face_plan(c, nullptr); // a plan belongs to the mesh it was built on
     const int64_t len = E * c->nd;
     (void)hipFree(c->d_h);
     (void)hipFree(c->d_f);

@@ -134,6 +134,10 @@ hipError_t launch_max_reduce(const double *partial, int64_t npart, double *out, 
 hipError_t launch_adaptive_dt(const double *cmax, double dx_min, double cfl, double dt_min,
                               double dt_max, double *dt, double *t, hipStream_t s);
 
+// Face term of an attached face plan (faces_api.cpp): dU += the plan's
+// conforming and mortar contributions.
+drf_status face_term(const drf_face_plan *p, const void *U, void *dU, int layout, hipStream_t s);
+
 // Mode split (modesplit.hip)
 hipError_t launch_mode_split(int order, int64_t E, double dt3, int nsub, double *full,
                              View<const double> h, View<const double> f, double f0,
@@ -164,4 +168,6 @@ struct drf_ctx {
     // volume-kernel launch shape (drf_set_launch / drf_autotune)
     int wg_per_cu = 0;
     int xcd_remap = 1;
+    // face term added to every right-hand side (drf_set_face_plan), or null
+    drf_face_plan *faces = nullptr;
 };

@@ -183,15 +183,7 @@ __device__ void pe_lf5(const double *L, const double *R, const double *n, double
 __device__ __forceinline__ double scalar_flux(int kind, double a, double b, const double *n,
                                               const FluxParams &p)
 {
-    const double vn = p.v[0] * n[0] + p.v[1] * n[1] + p.v[2] * n[2];
-    if (kind == DRF_FLUX_UPWIND)
-        return vn * ((vn >= 0) ? a : b);
-    double FnL = 0.0, FnR = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        FnL += (p.v[d] * a) * n[d];
-        FnR += (p.v[d] * b) * n[d];
-    }
-    return 0.5 * (FnL + FnR) - 0.5 * p.v[3] * (b - a);
+    return scalar_flux_t<double>(kind, a, b, n, p.v);
 }
 
 __global__ void __launch_bounds__(256)

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "drf_internal.hpp"
 #include "drifter_hip/drf.h"
 
 namespace drf {
@@ -37,6 +38,26 @@ __host__ __device__ inline int flux_num_vars(int kind)
     }
 }
 
+// Scalar numerical flux carried through ConformingInterface / MortarSpace
+// (U_L(1), U_R(1): src/dg/mortar.cpp:236-242, 330-336): UpwindFlux
+// (include/flux/numerical_flux.hpp:92-107) or the LaxFriedrichsFlux of the
+// advective flux F(U) = v U (:57-75).  v = {vx, vy, vz, lambda}.  No FMA
+// contraction, so every product and sum rounds as the reference's sequence.
+template <class T>
+__host__ __device__ inline T scalar_flux_t(int kind, T a, T b, const T *n, const T *v)
+{
+#pragma clang fp contract(off)
+    const T vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+    if (kind == DRF_FLUX_UPWIND)
+        return vn * ((vn >= T(0)) ? a : b);
+    T FnL = T(0), FnR = T(0);
+    for (int d = 0; d < 3; ++d) {
+        FnL += (v[d] * a) * n[d];
+        FnR += (v[d] * b) * n[d];
+    }
+    return T(0.5) * (FnL + FnR) - T(0.5) * v[3] * (b - a);
+}
+
 // Volume node of face DOF it + n jt on face `face` (0..5 = -xi, +xi, -eta,
 // +eta, -zeta, +zeta): the normal index sits at its boundary value and the
 // two tangent indices run over the remaining axes in increasing axis order
@@ -53,6 +74,29 @@ __host__ __device__ inline int face_node(int n, int face, int it, int jt)
     idx[t2] = jt;
     return idx[0] + n * (idx[1] + n * idx[2]);
 }
+
+// Arguments of the face-term kernels (faces.hip), by value: the state U and
+// tendencies dU in one strided layout (node d = i + n L of element e, field
+// f at f fs + e es + L ls + i is), the 1D LGL weights of the inverse mass
+// matrix, the conforming face rule and the scalar flux.
+template <class T> struct FaceArgs {
+    const T *U;
+    T *dU;
+    int64_t fs, es, ls, is;
+    T w[kMaxN];     // 1D LGL weights of the element basis
+    T rw[kMaxFaceQ]; // 1D weights of the conforming face rule (nq1 points)
+    int nq1;
+    int kind;        // DRF_FLUX_UPWIND or DRF_FLUX_LF_ADV
+    T p[4];          // vx, vy, vz, lambda
+};
+
+template <class T>
+hipError_t launch_face_conforming(int n, bool lift, int64_t nif, const int64_t *list,
+                                  const FaceArgs<T> &a, T *out_l, T *out_r, hipStream_t s);
+template <class T>
+hipError_t launch_face_mortar(int n, int nfine, bool lift, int64_t nif, const int64_t *list,
+                              const T *ops, const FaceArgs<T> &a, T *out_c, T *out_f,
+                              hipStream_t s);
 
 // Number of fine faces of a connection type (0 for boundary / invalid).
 inline int mortar_num_fine(int type)

@@ -167,6 +167,44 @@ int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
     return cur;
 }
 
+// Every leaf across face f of node id that shares a piece of the face of
+// positive area, in depth-first (element) order: a descent from the root
+// into the children that reach across the face plane and overlap the face
+// on both tangent axes (the 2:1 mortar connectivity, not the reference's).
+void adjacent_leaves(const drf_mesh &m, int64_t id, int f, std::vector<int64_t> &out)
+{
+    out.clear();
+    const double *b = m.nodes[id].b;
+    const int a = f / 2, t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+    const double plane = (f % 2) ? b[2 * a + 1] : b[2 * a];
+    const double tol = 1e-12 * (std::fabs(plane) + (b[2 * a + 1] - b[2 * a]));
+    auto reaches = [&](const double *x) {
+        if (f % 2) // the neighbour lies above the plane and touches it from there
+            return x[2 * a] <= plane + tol && x[2 * a + 1] > plane + tol;
+        return x[2 * a + 1] >= plane - tol && x[2 * a] < plane - tol;
+    };
+    auto overlaps = [&](const double *x, int t) {
+        const double lo = std::max(x[2 * t], b[2 * t]), hi = std::min(x[2 * t + 1], b[2 * t + 1]);
+        return hi - lo > 1e-12 * (b[2 * t + 1] - b[2 * t]);
+    };
+    std::vector<int64_t> stack{0};
+    while (!stack.empty()) {
+        const int64_t cur = stack.back();
+        stack.pop_back();
+        const Node &x = m.nodes[cur];
+        if (cur == id || !reaches(x.b) || !overlaps(x.b, t1) || !overlaps(x.b, t2))
+            continue;
+        if (!x.nchild) {
+            const double side = (f % 2) ? x.b[2 * a] : x.b[2 * a + 1];
+            if (std::fabs(side - plane) <= tol)
+                out.push_back(cur);
+            continue;
+        }
+        for (int c = x.nchild - 1; c >= 0; --c)
+            stack.push_back(x.child0 + c);
+    }
+}
+
 // fn(i0, i1) over [0, n) in contiguous chunks on up to 16 threads (one chunk
 // below 4,096 items); fn must only read shared state and write its own range.
 template <class F> void parallel_chunks(int64_t n, F fn)
@@ -399,8 +437,12 @@ drf_status drf_mesh_element_bounds(const drf_mesh *m, int64_t elem, double *b6)
     return DRF_OK;
 }
 
-drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *count,
-                                     int64_t *elems, int8_t *faces, int8_t *subs)
+} // extern "C"
+
+namespace {
+
+drf_status face_connections(const drf_mesh *m, bool at_level, uint8_t *type, int8_t *count,
+                            int64_t *elems, int8_t *faces, int8_t *subs)
 {
     if (!m || !type || !count || !elems || !faces || !subs)
         return drf::fail(DRF_ERR_ARG, "null argument");
@@ -424,10 +466,30 @@ drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *c
                                              : nd.b[2 * a] <= m->dom[2 * a] + 1e-12;
             if (on_boundary)
                 continue;
+            const int8_t opp = (int8_t)((f % 2 == 0) ? f + 1 : f - 1);
+            if (at_level) {
+                std::vector<int64_t> adj;
+                adjacent_leaves(*m, id, f, adj);
+                if (adj.empty())
+                    continue;
+                const Node &x = m->nodes[adj[0]];
+                const int t1 = (a == 0) ? 1 : 0, t2 = (a == 2) ? 1 : 2;
+                const bool covers = adj.size() == 1 && x.b[2 * t1] <= nd.b[2 * t1] &&
+                                    x.b[2 * t1 + 1] >= nd.b[2 * t1 + 1] &&
+                                    x.b[2 * t2] <= nd.b[2 * t2] &&
+                                    x.b[2 * t2 + 1] >= nd.b[2 * t2 + 1];
+                type[k] = covers ? 0 : (uint8_t)conn_type(nd.lev, x.lev, f);
+                count[k] = (int8_t)std::min<size_t>(adj.size(), 4);
+                for (size_t s = 0; s < adj.size() && s < 4; ++s) {
+                    elems[4 * k + s] = m->nodes[adj[s]].leaf;
+                    faces[4 * k + s] = opp;
+                    subs[4 * k + s] = (int8_t)s;
+                }
+                continue;
+            }
             const int64_t nb = neighbour_node(*m, id, f);
             if (nb < 0)
                 continue;
-            const int8_t opp = (int8_t)((f % 2 == 0) ? f + 1 : f - 1);
             if (!m->nodes[nb].nchild) {
                 type[k] = 0; // SameLevel, also when the neighbour is coarser (:331-338)
                 count[k] = 1;
@@ -463,6 +525,22 @@ drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *c
     }
     });
     return DRF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+drf_status drf_mesh_face_connections(const drf_mesh *m, uint8_t *type, int8_t *count,
+                                     int64_t *elems, int8_t *faces, int8_t *subs)
+{
+    return face_connections(m, false, type, count, elems, faces, subs);
+}
+
+drf_status drf_mesh_face_connections_2to1(const drf_mesh *m, uint8_t *type, int8_t *count,
+                                          int64_t *elems, int8_t *faces, int8_t *subs)
+{
+    return face_connections(m, true, type, count, elems, faces, subs);
 }
 
 } // extern "C"

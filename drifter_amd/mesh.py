@@ -54,15 +54,23 @@ class OctreeAdapter:
         check(self._l.drf_mesh_element_bounds(self._h, e, _lib.dptr(b)))
         return b
 
-    def build_face_connections(self):
+    def build_face_connections(self, two_to_one=False):
+        """OctreeAdapter::build_face_connections (octree_adapter.cpp:369-389)
+        as flat arrays (type, count, elems, faces, subs).  two_to_one: the
+        neighbour search stops at the element's own size, so a refined
+        neighbour gives its fine leaves and a 2:1 connection type (the
+        reference's search always reaches a leaf and reports SameLevel);
+        the face plans' mortars come from this mode."""
         E = self.num_elements()
         t = np.zeros((E, 6), dtype=np.uint8)
         c = np.zeros((E, 6), dtype=np.int8)
         el = np.zeros((E, 6, 4), dtype=np.int64)
         fa = np.zeros((E, 6, 4), dtype=np.int8)
         su = np.zeros((E, 6, 4), dtype=np.int8)
-        check(self._l.drf_mesh_face_connections(self._h, t.ctypes.data, c.ctypes.data,
-                                                el.ctypes.data, fa.ctypes.data, su.ctypes.data))
+        fn = (self._l.drf_mesh_face_connections_2to1 if two_to_one
+              else self._l.drf_mesh_face_connections)
+        check(fn(self._h, t.ctypes.data, c.ctypes.data, el.ctypes.data, fa.ctypes.data,
+                 su.ctypes.data))
         return t, c, el, fa, su
 
 

@@ -245,6 +245,19 @@ drf_status drf_mesh_element_bounds(const drf_mesh *mesh, int64_t elem, double *b
  * fine_face_ids, subface_indices; unused slots -1) */
 drf_status drf_mesh_face_connections(const drf_mesh *mesh, uint8_t *type, int8_t *count,
                                      int64_t *elems, int8_t *faces, int8_t *subs);
+/* The same arrays with the neighbour search of find_neighbor_same_or_coarser
+ * (octree_adapter.cpp:523-580) stopped at the first node of the element's
+ * own size.  The reference descends to a leaf every time, so its
+ * "refined neighbour" branch (get_neighbor :344-356, find_fine_neighbors
+ * :582-639) is never taken and a coarse face next to finer leaves is
+ * reported as SameLevel with the one leaf holding the probe point.  Here that
+ * branch is taken: a refined same-size neighbour gives its fine leaves
+ * sharing the face in breadth-first order (subface index = rank), typed by
+ * get_connection_type (face_connection.hpp:110-152).  Not a parity entry
+ * point: the 2:1 mortar faces of the face term (drf_face_plan_create) come
+ * from it. */
+drf_status drf_mesh_face_connections_2to1(const drf_mesh *mesh, uint8_t *type, int8_t *count,
+                                          int64_t *elems, int8_t *faces, int8_t *subs);
 
 /* ------------------------------------------------------------------ */
 /* Numerical fluxes (NumericalFlux::flux, include/flux/numerical_flux.hpp:41)
@@ -323,6 +336,75 @@ drf_status drf_verify_mortar_conservation(const drf_mortar *m, const double *rhs
 drf_status drf_mortar_conservation_dev(const drf_mortar *m, int64_t n_interfaces,
                                        const void *rhs_c, const void *rhs_f, const void *w_coarse,
                                        const void *w_fine, void *rel_error, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Face term of the DG right-hand side on an octree mesh               */
+/* (DG3DIntegration::compute_rhs face loop and inverse mass,           */
+/*  src/dg/operators_3d.cpp:255-281)                                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct drf_face_plan drf_face_plan;
+
+/* Interface lists of a mesh, grouped for the face kernels.  From a face
+ * connectivity (the arrays of drf_mesh_face_connections or
+ * drf_mesh_face_connections_2to1, E*6 and E*6*4), the elements' bounds (E*6)
+ * and directional levels (E*3):
+ *  - conforming interfaces: SameLevel pairs whose neighbour points back,
+ *    once each (from the lower element), normal = outward normal of that
+ *    element's face, flux as compute_interface_flux_conforming (:320-343)
+ *    computes it through ConformingInterface (src/dg/mortar.cpp:297-343,
+ *    N-point face rule included);
+ *  - 2:1 mortars: every non-SameLevel, non-Boundary connection from its
+ *    coarse side, as update_mortar_interfaces registers them
+ *    (src/amr/refinement.cpp:575-596), normal = outward normal of the coarse
+ *    face, flux as compute_interface_flux_nonconforming (:345-391) through
+ *    MortarSpace::compute_mortar_flux (src/dg/mortar.cpp:212-258), the fine
+ *    faces put on the sub-faces of compute_subface_bounds (:27-78) they
+ *    cover;
+ *  - a SameLevel face against a coarser leaf adds nothing itself (the coarse
+ *    side's mortar covers it; with the reference's connectivity nothing
+ *    does).
+ * The same scalar flux (kind DRF_FLUX_UPWIND or DRF_FLUX_LF_ADV, params as
+ * for drf_flux_points) is applied to each of the five fields.  Interfaces are
+ * grouped by normal axis, kind (DRF_CONN_SAME_LEVEL for conforming faces,
+ * else the mortar's connection type) and level (the largest directional
+ * level of the element owning the interface); each (axis, kind) runs as one
+ * persistent kernel over its level groups.  Orders 1-4.
+ * DRF_ERR_UNSUPPORTED when a mortar's fine faces do not tile its coarse face
+ * as its type says (the 1:3 fallback of get_connection_type);
+ * DRF_ERR_ARG when a face would belong to two interfaces.  The plan keeps
+ * device copies of its lists (the context's device and precision). */
+drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double *bounds,
+                                const int32_t *levels, const uint8_t *type, const int8_t *count,
+                                const int64_t *elems, const int8_t *faces, int kind,
+                                const double *params, drf_face_plan **out);
+void drf_face_plan_destroy(drf_face_plan *plan);
+drf_status drf_face_plan_info(const drf_face_plan *plan, int *n_groups, int64_t *n_conforming,
+                              int64_t *n_mortar);
+/* per group, in launch order: desc[5g .. 5g+4] = axis, kind, level, first
+ * index into its kind's list, count */
+drf_status drf_face_plan_groups(const drf_face_plan *plan, int64_t *desc);
+/* the lists in group order: conforming[4c ..] = left element, left face,
+ * right element, right face; mortar[6m ..] = coarse element, coarse face,
+ * fine elements of sub-faces 0..3 (-1 past the type's count) */
+drf_status drf_face_plan_lists(const drf_face_plan *plan, int64_t *conforming, int64_t *mortar);
+/* dU += the face term: each interface's face vectors lifted to the element
+ * nodes (I_f^T) and multiplied by the inverse LGL mass matrix 1/(w_i w_j w_k)
+ * (DG3DElementOperator::mass_inv, operators_3d.cpp:52-55;
+ * basis_hexahedron.cpp:393-420).  U, dU device pointers in `layout`, element
+ * type of the plan's context. */
+drf_status drf_face_term_dev(drf_face_plan *plan, const void *U, void *dU, int layout,
+                             void *stream);
+/* The face vectors of one group before lifting: conforming groups write
+ * rhs_l to out_a and rhs_r to out_b, [interface][field][(N+1)^2]; mortar
+ * groups write rhs_c to out_a [interface][field][(N+1)^2] and rhs_f to out_b
+ * [interface][field][fine][(N+1)^2]. */
+drf_status drf_face_group_fluxes_dev(drf_face_plan *plan, int group, const void *U, int layout,
+                                     void *out_a, void *out_b, void *stream);
+/* Attach a plan (NULL detaches): drf_rhs, drf_rhs_dev and the un-fused
+ * Runge-Kutta steps then add its face term to every right-hand side; the
+ * fused steps refuse (DRF_ERR_UNSUPPORTED).  drf_initialize detaches. */
+drf_status drf_set_face_plan(drf_ctx *ctx, drf_face_plan *plan);
 
 /* ------------------------------------------------------------------ */
 /* Error estimator and element marking (GradientErrorEstimator,        */

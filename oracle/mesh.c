@@ -18,7 +18,15 @@
  * that one fine leaf; the "refined neighbour" branch (:344-356) is reached
  * only when the descent finds no child containing the point.
  *
- * Pinned by (tests/test_oracle_mesh.py): the reference's count tests
+ * orc_mesh_connections_2to1 (not the reference's): every face lists the
+ * leaves across it that share a piece of it, found by a descent over the
+ * tree's boxes; the face plans' 2:1 mortars come from it.
+ *
+ * Pinned by (tests/test_oracle_mesh.py): on random anisotropic refine +
+ * balance meshes, every FaceConnection against a linear scan (the leaf
+ * holding the probe point with the smallest depth-first number, for the
+ * reference's connectivity; the leaves sharing a piece of the face, for the
+ * 2:1 one); the reference's count tests
  * (tests/unit/test_amr.cpp, tests/integration/test_mesh.cpp: 2x2x2 = 8 and
  * boundary/interior mix of 2x2x1), brute-force geometric adjacency on uniform
  * meshes (every interior face's neighbour shares the face and points back),
@@ -378,9 +386,55 @@ void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t 
     }
 }
 
+/* The 2:1 mortar connectivity's neighbours: every leaf on the other side of
+ * face f of leaf id whose face overlaps it with positive area, in depth-first
+ * order (recursion over the children in creation order). */
+static void adjacent_rec(const orc_mesh *m, int64_t cur, int64_t id, int f, int64_t *out, int maxout,
+                         int *cnt)
+{
+    const double *b = m->nodes[id].b, *x = m->nodes[cur].b;
+    const int a = f / 2;
+    const double plane = (f % 2) ? b[2 * a + 1] : b[2 * a];
+    const double tol = 1e-12 * (fabs(plane) + (b[2 * a + 1] - b[2 * a]));
+    if (cur == id)
+        return;
+    if (f % 2) {
+        if (!(x[2 * a] <= plane + tol && x[2 * a + 1] > plane + tol))
+            return;
+    } else if (!(x[2 * a + 1] >= plane - tol && x[2 * a] < plane - tol)) {
+        return;
+    }
+    for (int t = 0; t < 3; ++t) {
+        if (t == a)
+            continue;
+        const double lo = x[2 * t] > b[2 * t] ? x[2 * t] : b[2 * t];
+        const double hi = x[2 * t + 1] < b[2 * t + 1] ? x[2 * t + 1] : b[2 * t + 1];
+        if (!(hi - lo > 1e-12 * (b[2 * t + 1] - b[2 * t])))
+            return;
+    }
+    if (!m->nodes[cur].nchild) {
+        const double side = (f % 2) ? x[2 * a] : x[2 * a + 1];
+        if (fabs(side - plane) <= tol) {
+            if (*cnt < maxout)
+                out[*cnt] = cur;
+            ++*cnt;
+        }
+        return;
+    }
+    for (int c = 0; c < m->nodes[cur].nchild; ++c)
+        adjacent_rec(m, m->nodes[cur].child[c], id, f, out, maxout, cnt);
+}
+
+static int adjacent(const orc_mesh *m, int64_t id, int f, int64_t *out, int maxout)
+{
+    int cnt = 0;
+    adjacent_rec(m, 0, id, f, out, maxout, &cnt);
+    return cnt;
+}
+
 /* get_neighbor :259-359 and build_face_connections :369-389 */
-void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
-                          int8_t *faces, int8_t *subs)
+static void connections(const orc_mesh *m, int at_level, uint8_t *type, int8_t *count,
+                        int64_t *elems, int8_t *faces, int8_t *subs)
 {
     for (int64_t e = 0; e < m->nl; ++e) {
         const int64_t id = m->leaves[e];
@@ -405,10 +459,26 @@ void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64
             }
             if (at)
                 continue;
+            const int opp = (f % 2 == 0) ? f + 1 : f - 1;
+            if (at_level) {
+                int64_t adj[64];
+                const int cnt = adjacent(m, id, f, adj, 64);
+                if (!cnt)
+                    continue;
+                const onode *x = &m->nodes[adj[0]];
+                type[k] = (uint8_t)(cnt == 1 ? ORC_CONN_SAME
+                                             : connection_type(m->nodes[id].lev, x->lev, f));
+                count[k] = (int8_t)(cnt < 4 ? cnt : 4);
+                for (int s = 0; s < cnt && s < 4; ++s) {
+                    elems[4 * k + s] = m->nodes[adj[s]].leaf;
+                    faces[4 * k + s] = (int8_t)opp;
+                    subs[4 * k + s] = (int8_t)s;
+                }
+                continue;
+            }
             const int64_t nb = neighbor_same_or_coarser(m, id, f);
             if (nb < 0)
                 continue;
-            const int opp = (f % 2 == 0) ? f + 1 : f - 1;
             if (!m->nodes[nb].nchild) {
                 type[k] = ORC_CONN_SAME; /* :331-338: SameLevel either way */
                 count[k] = 1;
@@ -429,4 +499,20 @@ void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64
             }
         }
     }
+}
+
+void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
+                          int8_t *faces, int8_t *subs)
+{
+    connections(m, 0, type, count, elems, faces, subs);
+}
+
+/* The 2:1 mortar connectivity: every face lists the leaves across it that
+ * share a piece of it (depth-first order); one leaf is SameLevel, several are
+ * typed by get_connection_type from the first, as get_neighbor's
+ * refined-neighbour branch (:344-356) would type them. */
+void orc_mesh_connections_2to1(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
+                               int8_t *faces, int8_t *subs)
+{
+    connections(m, 1, type, count, elems, faces, subs);
 }

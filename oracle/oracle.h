@@ -116,6 +116,9 @@ void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t 
  * face ids and sub-face indices (unused slots -1) */
 void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
                           int8_t *faces, int8_t *subs);
+/* the 2:1 mortar connectivity (the descent stopped at the element's size) */
+void orc_mesh_connections_2to1(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,
+                               int8_t *faces, int8_t *subs);
 
 /* ---- A14-A16, A18: fluxes, conforming interfaces, mortars ---- */
 enum {
@@ -141,6 +144,17 @@ int orc_mortar_operators(int order, int type, double *M, double *Minv, double *P
 /* MortarSpace::compute_mortar_flux (scalar); Uf / rhs_f are nfine*(N+1)^2 */
 int orc_mortar_flux(int order, int type, int kind, const double *p, const double *Uc,
                     const double *Uf, const double *n, double *rhs_c, double *rhs_f);
+
+/* ---- face term of the DG right-hand side (faces.c) ---- */
+/* interface lists: conf [nc][4], mort [nm][6], groups [ng][5] (sizes up to
+ * 6 E); returns -1 for a connection without a 2:1 mortar */
+int orc_face_lists(int64_t E, const double *bounds, const int32_t *levels, const uint8_t *type,
+                   const int8_t *count, const int64_t *elems, int64_t *conf, int64_t *nconf,
+                   int64_t *mort, int64_t *nmort, int64_t *groups, int64_t *ngroups);
+/* dU += M^-1 (sum of the lifted face vectors), packed layout */
+int orc_face_term(int order, int64_t E, int64_t nconf, const int64_t *conf, int64_t nmort,
+                  const int64_t *mort, const int *mort_type, int kind, const double *p,
+                  const double *U, double *dU);
 
 /* ---- A19: GradientErrorEstimator + mark_elements ---- */
 /* GradientErrorEstimator::estimate for E elements of one nodal field

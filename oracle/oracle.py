@@ -66,6 +66,10 @@ def load():
         "orc_mesh_num_elements": (C.c_int64, [C.c_void_p]),
         "orc_mesh_elements": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_mesh_connections": (None, [C.c_void_p] * 6),
+        "orc_mesh_connections_2to1": (None, [C.c_void_p] * 6),
+        "orc_face_lists": (C.c_int, [C.c_int64] + [C.c_void_p] * 11),
+        "orc_face_term": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.c_void_p, C.c_void_p, C.c_int, _PD, _PD, _PD]),
         "orc_flux": (C.c_int, [C.c_int, _PD, _PD, _PD, _PD, _PD]),
         "orc_face_map": (None, [C.c_int, C.c_int, C.c_void_p]),
         "orc_conforming_flux": (C.c_int, [C.c_int, C.c_int, _PD, _PD, _PD, _PD, _PD, _PD]),
@@ -244,16 +248,51 @@ class Mesh:
         self._l.orc_mesh_elements(self.h, b.ctypes.data, lv.ctypes.data, mo.ctypes.data)
         return b, lv, mo
 
-    def connections(self):
+    def connections(self, two_to_one=False):
         E = self.num_elements
         t = np.zeros((E, 6), dtype=np.uint8)
         c = np.zeros((E, 6), dtype=np.int8)
         el = np.zeros((E, 6, 4), dtype=np.int64)
         fa = np.zeros((E, 6, 4), dtype=np.int8)
         su = np.zeros((E, 6, 4), dtype=np.int8)
-        self._l.orc_mesh_connections(self.h, t.ctypes.data, c.ctypes.data, el.ctypes.data,
-                                     fa.ctypes.data, su.ctypes.data)
+        fn = self._l.orc_mesh_connections_2to1 if two_to_one else self._l.orc_mesh_connections
+        fn(self.h, t.ctypes.data, c.ctypes.data, el.ctypes.data, fa.ctypes.data, su.ctypes.data)
         return t, c, el, fa, su
+
+
+def face_lists(bounds, levels, conn):
+    """Interface lists of a connectivity (oracle/faces.c): conforming (nc, 4),
+    mortar (nm, 6), groups (ng, 5)."""
+    t, c, el = (np.ascontiguousarray(conn[0], dtype=np.uint8),
+                np.ascontiguousarray(conn[1], dtype=np.int8),
+                np.ascontiguousarray(conn[2], dtype=np.int64))
+    b = np.ascontiguousarray(bounds, dtype=np.float64)
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    E = b.shape[0]
+    cf = np.zeros((6 * E, 4), dtype=np.int64)
+    mo = np.zeros((6 * E, 6), dtype=np.int64)
+    gr = np.zeros((6 * E, 5), dtype=np.int64)
+    nc, nm, ng = C.c_int64(), C.c_int64(), C.c_int64()
+    rc = load().orc_face_lists(E, b.ctypes.data, lv.ctypes.data, t.ctypes.data, c.ctypes.data,
+                               el.ctypes.data, cf.ctypes.data, C.byref(nc), mo.ctypes.data,
+                               C.byref(nm), gr.ctypes.data, C.byref(ng))
+    if rc:
+        raise ValueError("a connection without a 2:1 mortar")
+    return cf[:nc.value].copy(), mo[:nm.value].copy(), gr[:ng.value].copy()
+
+
+def face_term(order, E, conf, mort, mort_type, kind, p, U, dU=None):
+    """dU + the face term (packed layout); dU = 0 when None."""
+    out = np.zeros(E * 5 * (order + 1) ** 3) if dU is None else np.array(dU, dtype=np.float64)
+    conf = np.ascontiguousarray(conf, dtype=np.int64).reshape(-1, 4)
+    mort = np.ascontiguousarray(mort, dtype=np.int64).reshape(-1, 6)
+    mt = np.ascontiguousarray(mort_type, dtype=np.int32)
+    pp = np.zeros(4)
+    pp[: len(p)] = p
+    rc = load().orc_face_term(order, E, conf.shape[0], conf.ctypes.data, mort.shape[0],
+                              mort.ctypes.data, mt.ctypes.data, kind, _p(pp), _p(_v(U)), _p(out))
+    assert rc == 0
+    return out
 
 
 SW_LF, SW_HLLC, SW_ROE, SW_CENTRAL, PE_LF5, UPWIND, LF_ADV = range(7)

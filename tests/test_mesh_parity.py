@@ -1,7 +1,8 @@
 """Connectivity parity (A17): the product's octree (drf_mesh_*, host C++)
 against the oracle's restatement of OctreeAdapter, bit-exact on element
-bounds, levels, Morton codes and every FaceConnection field.  Host-only code:
-runs without a GPU."""
+bounds, levels, Morton codes and every FaceConnection field, of the
+reference's connectivity and of the 2:1 connectivity the face plans use.
+Host-only code: runs without a GPU."""
 import numpy as np
 import pytest
 
@@ -15,10 +16,11 @@ def _compare(prod, orc_mesh):
     assert np.array_equal(b1, b2)  # bit-exact FP64 bounds
     assert np.array_equal(l1, l2)
     assert np.array_equal(m1, m2)
-    c1 = prod.build_face_connections()
-    c2 = orc_mesh.connections()
-    for a, b in zip(c1, c2):
-        assert np.array_equal(a, b)
+    for two in (False, True):
+        c1 = prod.build_face_connections(two_to_one=two)
+        c2 = orc_mesh.connections(two_to_one=two)
+        for a, b in zip(c1, c2):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (2, 2, 1), (8, 8, 4), (5, 3, 2), (64, 64, 8)])

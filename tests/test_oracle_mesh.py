@@ -131,3 +131,86 @@ def test_coarse_to_fine_face_reports_the_leaf_at_the_centre(orc):
     inside = (b[:, 0] <= p[0]) & (p[0] <= b[:, 1]) & (b[:, 2] <= p[1]) & (p[1] <= b[:, 3])
     assert inside[nb]
     assert lv[nb][0] == lv[e0][0] + 1
+
+
+def _probe(bb, f):
+    """Centre of the same-size box across face f (octree_adapter.cpp:530-558)."""
+    p = 0.5 * (bb[0::2] + bb[1::2])
+    a = f // 2
+    lo, hi = bb[2 * a], bb[2 * a + 1]
+    p[a] = hi + 0.5 * (hi - lo) if f % 2 else lo - 0.5 * (hi - lo)
+    return p
+
+
+def _leaf_at(b, p):
+    """The leaf the reference's descent reaches: among the leaves whose
+    bounds hold p (inclusive), the first in depth-first order, i.e. the
+    smallest element number (at every level the descent takes the first
+    child in creation order that holds p, and elements are numbered depth
+    first with children in creation order)."""
+    ins = np.all((b[:, 0::2] <= p) & (p <= b[:, 1::2]), axis=1)
+    return int(np.nonzero(ins)[0][0])
+
+
+def _random_refined(orc, seed, rounds=3):
+    rng = np.random.default_rng(seed)
+    m = orc.Mesh(0.0, 1.0, 0.0, 2.0, -1.0, 0.0)
+    m.build_uniform(4, 4, 2)
+    for _ in range(rounds):
+        E = m.num_elements
+        sel = np.unique(rng.integers(0, E, size=max(1, E // 5)))
+        m.refine(sel, rng.integers(1, 8, size=sel.size))
+    return m
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_refined_connectivity_brute_force(orc, seed):
+    """After random anisotropic refine + balance, every FaceConnection of the
+    reference's connectivity is the leaf holding the probe point found by a
+    linear scan (SameLevel, one neighbour, opposite face), and every face of
+    the 2:1 connectivity lists exactly the leaves sharing a piece of the face
+    (geometric search), one covering leaf as SameLevel."""
+    m = _random_refined(orc, seed)
+    b, lv, _ = m.elements()
+    dom = np.array([0.0, 1.0, 0.0, 2.0, -1.0, 0.0])
+    t, c, el, fa, _ = m.connections()
+    t2, c2, el2, fa2, _ = m.connections(two_to_one=True)
+    nmortar = 0
+    for e in range(b.shape[0]):
+        for f in range(6):
+            a = f // 2
+            on = b[e, 2 * a + 1] >= dom[2 * a + 1] - 1e-12 if f % 2 else \
+                b[e, 2 * a] <= dom[2 * a] + 1e-12
+            if on:
+                assert t[e, f] == orc.CONN_BOUNDARY and t2[e, f] == orc.CONN_BOUNDARY
+                continue
+            opp = f + 1 if f % 2 == 0 else f - 1
+            assert t[e, f] == orc.CONN_SAME and c[e, f] == 1 and fa[e, f, 0] == opp
+            assert el[e, f, 0] == _leaf_at(b, _probe(b[e], f))
+            br = sorted(_brute_neighbours(b, e, f))
+            assert br
+            k = c2[e, f]
+            assert sorted(el2[e, f, :k].tolist()) == br[:4]
+            assert (fa2[e, f, :k] == opp).all()
+            if len(br) == 1:
+                assert t2[e, f] == orc.CONN_SAME
+            else:
+                nmortar += 1
+                assert t2[e, f] != orc.CONN_SAME
+    assert nmortar > 0
+
+
+def test_two_to_one_connectivity_a_coarse_face_lists_its_fine_leaves(orc):
+    """The case the reference's descent cannot reach: a face whose same-size
+    neighbour is refined lists the fine leaves across it (Fine2x2 here), and
+    the fine leaves point back at the coarse element as SameLevel."""
+    m = _mesh(orc, 2, 2, 1)
+    m.refine([1], [3])  # element 1 split in XY: its -x face now holds two leaves
+    b, lv, _ = m.elements()
+    t, c, el, fa, su = m.connections(two_to_one=True)
+    # element 0 (x in [0, 5000], y in [0, 5000]) across +x: two leaves, split in y only
+    assert c[0, 1] == 2 and t[0, 1] == orc.CONN_FINE2X1
+    fine = el[0, 1, :2]
+    assert set(fine) == set(_brute_neighbours(b, 0, 1))
+    for x in fine:
+        assert t[x, 0] == orc.CONN_SAME and el[x, 0, 0] == 0
