@@ -114,6 +114,8 @@ SIGNATURES = {
     "drf_mlp_classify_dev": (_I, [_L, _P, _D, _D, _P, _P]),
     "drf_mlp_loss_grad_dev": (_I, [_P, _L, _P, _P, _P, _P, _P]),
     "drf_mlp_train_step_dev": (_I, [_P, _L, _P, _P, _D, _P, _P]),
+    "drf_hydrostatic_pressure_dev": (_I, [_I, _L, _D, _P, _P, _L, _P, _P]),
+    "drf_pressure_gradient_dev": (_I, [_I, _L, _I, _I, _D, _D] + [_P] * 8 + [_L, _P, _P, _P]),
     "drf_sigma_metrics_dev": (_I, [_L] + [_P] * 14),
     "drf_sigma_ops_metrics_dev": (_I, [_I, _L] + [_P] * 7 + [_L] + [_P] * 7),
     "drf_sigma_horizontal_gradient_dev": (_I, [_I, _L, _P, _P, _P, _L, _P, _P, _P]),

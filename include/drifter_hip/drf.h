@@ -338,6 +338,36 @@ drf_status drf_mortar_conservation_dev(const drf_mortar *m, int64_t n_interfaces
                                        const void *w_fine, void *rel_error, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Hydrostatic pressure and pressure gradient in sigma coordinates     */
+/* (PressureGradientElement, include/physics/pressure_gradient.hpp:    */
+/*  46-367; the reference header does not compile, :183)               */
+/* ------------------------------------------------------------------ */
+
+#define DRF_PG_STANDARD 0         /* baroclinic_gradient_standard, :252-294 */
+#define DRF_PG_DENSITY_JACOBIAN 1 /* baroclinic_gradient_density_jacobian, :296-338 */
+
+/* hydrostatic_pressure (:233-250) with the vertical direction on the
+ * element's zeta index k: p(i,j,k) = sum_{k' >= k} (w_k'/2) g rho H, summed
+ * from k up.  Device arrays, element e at e * elem_stride (>= (N+1)^3).
+ * Orders 1-7. */
+drf_status drf_hydrostatic_pressure_dev(int order, int64_t n_elem, double g, const void *rho,
+                                        const void *H, int64_t elem_stride, void *p,
+                                        void *stream);
+/* full_gradient (:340-367): parts 1 = barotropic -g H grad(eta) (:213-231),
+ * 2 = baroclinic of `method`, 3 = their sum.  Standard: dp/dx|_z = D_xi p +
+ * (D_zeta p)(sigma dH/dx - deta/dx)/H, dH/dx = deta/dx + dh/dx, pg = -dp/dx|_z
+ * / rho0 (deta_dx, deta_dy, dh_dx, dh_dy may be NULL: 0; the reference
+ * leaves deta/dx at 0).  Density Jacobian: z = eta + sigma H, J_x = rho_xi
+ * z_zeta - rho_zeta z_xi, pg = -(g/rho0) sum_{k' <= k} (w_k'/2) J.
+ * Derivatives in reference coordinates (DG3DElementOperator::
+ * gradient_reference, src/dg/operators_3d.cpp:58-69). */
+drf_status drf_pressure_gradient_dev(int order, int64_t n_elem, int method, int parts, double g,
+                                     double rho0, const void *eta, const void *rho, const void *H,
+                                     const void *sigma, const void *deta_dx, const void *deta_dy,
+                                     const void *dh_dx, const void *dh_dy, int64_t elem_stride,
+                                     void *pg_x, void *pg_y, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Face term of the DG right-hand side on an octree mesh               */
 /* (DG3DIntegration::compute_rhs face loop and inverse mass,           */
 /*  src/dg/operators_3d.cpp:255-281)                                   */

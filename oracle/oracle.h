@@ -145,6 +145,15 @@ int orc_mortar_operators(int order, int type, double *M, double *Minv, double *P
 int orc_mortar_flux(int order, int type, int kind, const double *p, const double *Uc,
                     const double *Uf, const double *n, double *rhs_c, double *rhs_f);
 
+/* ---- hydrostatic pressure and pressure gradient (pressure.c) ---- */
+int orc_hydrostatic_pressure(int order, int64_t E, double g, const double *rho, const double *H,
+                             double *p);
+/* parts: 1 barotropic, 2 baroclinic, 3 both; method 0 standard, 1 density Jacobian */
+int orc_pressure_gradient(int order, int64_t E, int method, int parts, double g, double rho0,
+                          const double *eta, const double *rho, const double *H,
+                          const double *sigma, const double *deta_dx, const double *deta_dy,
+                          const double *dh_dx, const double *dh_dy, double *pg_x, double *pg_y);
+
 /* ---- face term of the DG right-hand side (faces.c) ---- */
 /* interface lists: conf [nc][4], mort [nm][6], groups [ng][5] (sizes up to
  * 6 E); returns -1 for a connection without a 2:1 mortar */

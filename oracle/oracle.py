@@ -68,6 +68,9 @@ def load():
         "orc_mesh_connections": (None, [C.c_void_p] * 6),
         "orc_mesh_connections_2to1": (None, [C.c_void_p] * 6),
         "orc_face_lists": (C.c_int, [C.c_int64] + [C.c_void_p] * 11),
+        "orc_hydrostatic_pressure": (C.c_int, [C.c_int, C.c_int64, C.c_double, _PD, _PD, _PD]),
+        "orc_pressure_gradient": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double,
+                                            C.c_double] + [C.c_void_p] * 10),
         "orc_face_term": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
                                     C.c_void_p, C.c_void_p, C.c_int, _PD, _PD, _PD]),
         "orc_flux": (C.c_int, [C.c_int, _PD, _PD, _PD, _PD, _PD]),
@@ -524,3 +527,28 @@ def verify_mortar_conservation(rc, rf, wc, wf):
     rc, rf, wc, wf = (_v(a) for a in (rc, rf, wc, wf))
     return load().orc_verify_mortar_conservation(rc.size, rf.shape[0], _p(rc), _p(rf), _p(wc),
                                                  _p(wf))
+
+
+def hydrostatic_pressure(order, rho, H, g=9.81):
+    rho, H = _v(rho), _v(H)
+    nd = (order + 1) ** 3
+    p = np.zeros_like(rho)
+    assert load().orc_hydrostatic_pressure(order, rho.size // nd, g, _p(rho), _p(H), _p(p)) == 0
+    return p
+
+
+def pressure_gradient(order, method, parts, eta, rho, H, sigma, deta_dx=None, deta_dy=None,
+                      dh_dx=None, dh_dy=None, g=9.81, rho0=1025.0):
+    """(pg_x, pg_y) element-major; method 0 standard, 1 density Jacobian;
+    parts 1 barotropic, 2 baroclinic, 3 both."""
+    arrs = [_v(a) for a in (eta, rho, H, sigma)]
+    opt = [None if a is None else _v(a) for a in (deta_dx, deta_dy, dh_dx, dh_dy)]
+    nd = (order + 1) ** 3
+    px = np.zeros_like(arrs[0])
+    py = np.zeros_like(arrs[0])
+    ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    rc = load().orc_pressure_gradient(order, arrs[0].size // nd, method, parts, g, rho0,
+                                      *(a.ctypes.data for a in arrs), *(ptr(a) for a in opt),
+                                      px.ctypes.data, py.ctypes.data)
+    assert rc == 0
+    return px, py
