@@ -158,37 +158,37 @@ def cpu_baseline(case, label, target_s=10.0):
 
 def copy_probe(steps, warmup):
     """Plain device copies with the RHS's field sizes (64x64x8 elements,
-    125 nodes, FP64): (a) 5 fields copied, 1 read + 1 write per value;
-    (b) 5 fields plus a broadcast 6th field added into 5 outputs, the volume
-    kernel's 6-in/5-out pattern.  Their HBM counters per dispatch give the
-    scale of FETCH_SIZE / WRITE_SIZE, and their time the copy ceiling."""
+    125 nodes, FP64): (a) 5 fields copied, 1 read + 1 write per value (the
+    only copy-engine kernel in this run: every other buffer is filled, not
+    cloned, so the per-dispatch HBM counters of that kernel are of one size,
+    163,840,000 B read and written, and give the scale of FETCH_SIZE /
+    WRITE_SIZE); (b) 5 fields plus a broadcast 6th field added into 5
+    outputs, the volume kernel's 6-in/5-out pattern, on one buffer set and
+    (c, as a subtraction, a kernel of its own) cycling over four sets so that
+    every launch reads from HBM: the copy ceiling."""
     import torch
 
     nn = 64 * 64 * 8 * 125
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream()
-    src = torch.ones(6 * nn, dtype=torch.float64, device=dev)
-    dst = torch.empty(5 * nn, dtype=torch.float64, device=dev)
+    srcs = [torch.full((6 * nn,), 1.0 + k, dtype=torch.float64, device=dev) for k in range(4)]
+    dsts = [torch.zeros(5 * nn, dtype=torch.float64, device=dev) for _ in range(4)]
+    src, dst = srcs[0], dsts[0]
     a = time_launches(lambda: dst.copy_(src[: 5 * nn]), steps, warmup, stream)
     b = time_launches(lambda: torch.add(src[: 5 * nn].view(5, nn), src[5 * nn:].view(1, nn),
                                         out=dst.view(5, nn)), steps, warmup, stream)
-    # (c) as (b), cycling over four source/destination sets (1.4 GB), so every
-    # launch reads from HBM rather than from the 256 MB last-level cache: the
-    # ceiling for the RHS value, which is timed the same way
-    srcs = [src] + [src.clone() for _ in range(3)]
-    dsts = [dst] + [torch.empty_like(dst) for _ in range(3)]
     it = [0]
 
-    def add_rot():
+    def sub_rot():
         k = it[0] % 4
         it[0] += 1
-        torch.add(srcs[k][: 5 * nn].view(5, nn), srcs[k][5 * nn:].view(1, nn),
+        torch.sub(srcs[k][: 5 * nn].view(5, nn), srcs[k][5 * nn:].view(1, nn),
                   out=dsts[k].view(5, nn))
 
-    c = time_launches(add_rot, steps, warmup, stream)
+    c = time_launches(sub_rot, steps, warmup, stream)
     ba, bb = 10 * nn * 8, 11 * nn * 8
     print(json.dumps({
-        "add_6_in_5_out_from_hbm": {"us": round(c * 1e3, 2), "bytes": bb,
+        "sub_6_in_5_out_from_hbm": {"us": round(c * 1e3, 2), "bytes": bb,
                                     "gb_s": round(bb / (c * 1e-3) / 1e9, 1),
                                     "sets": 4},
         "copy_5_fields": {"us": round(a * 1e3, 2), "bytes": ba,
@@ -371,6 +371,90 @@ def measure_coastline(args, stream):
         out[f"coastline_39280_n3_{name}_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
         out[f"coastline_39280_n3_{name}_us"] = round(ms * 1e3, 2)
     log(f"coastline N=3 RHS: {out}")
+    return out
+
+
+def measure_faces_coastline(args, stream):
+    """The batched adaptive DG right-hand side on the coastline octree
+    (39,280 elements, N=3): the face term of the 2:1 plan (103,116 conforming
+    interfaces and 3,012 Fine2x1 mortars in 18 (axis, kind, level) groups,
+    run as 6 persistent launches) alone, and the volume RHS with it attached,
+    FP64 and mixed, with the state read from HBM (8 rotating sets)."""
+    import torch
+
+    from drifter_amd import _lib, faces, synth
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    case, mesh = synth.config3()
+    E, nd = case.E, case.nd
+    b, lv, _ = mesh.elements()
+    conn = mesh.build_face_connections(two_to_one=True)
+    out = {}
+    for name, prec, dt in (("fp64", _lib.PREC_FP64, torch.float64),
+                           ("mixed", _lib.PREC_MIXED, torch.float32)):
+        s = PrimitiveEquationsSolver(3, prec)
+        s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+        plan = faces.FacePlan(s, b, lv, conn, kind=faces.LF_ADV, params=(0.3, -0.2, 0.1, 0.6))
+        U = torch.from_numpy(synth.to_lines(case.U, E, 4)).to("cuda").to(dt)
+        R = torch.empty_like(U)
+        ms_face = time_launches(lambda: plan.face_term(U, R, _lib.LAYOUT_LINES, stream), 50, 10,
+                                stream)
+        plan.attach()
+        ms = time_rotating(s, U, R, 8, args.steps, args.warmup, stream)
+        plan.detach()
+        ng, nc, nm = plan.info()
+        out[f"coastline_faces_{name}_face_term_us"] = round(ms_face * 1e3, 2)
+        out[f"coastline_faces_{name}_rhs_with_faces_us"] = round(ms * 1e3, 2)
+        out[f"coastline_faces_{name}_rhs_with_faces_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
+        out["coastline_face_plan_groups_conforming_mortars"] = [ng, nc, nm]
+        plan.close()
+        s.close()
+        del U, R
+    log(f"coastline RHS with faces: {out}")
+    return out
+
+
+def measure_faces_uniform(args, case, stream):
+    """The face term on the headline mesh (every face conforming: 3 groups,
+    765,952 interfaces) and the RHS with it, FP64 and mixed (FP32 volume and
+    surface kernels), with the state read from HBM."""
+    import torch
+
+    from drifter_amd import _lib, faces, synth
+    from drifter_amd.mesh import OctreeAdapter
+    from drifter_amd.solver import CoriolisParameter, PrimitiveEquationsSolver
+
+    E, nd = case.E, case.nd
+    m = OctreeAdapter(0.0, 100000.0, 0.0, 100000.0, -1.0, 0.0)
+    m.build_uniform(128, 128, 16)
+    b, lv, _ = m.elements()
+    conn = m.build_face_connections(two_to_one=True)
+    out = {}
+    for name, prec, dt in (("fp64", _lib.PREC_FP64, torch.float64),
+                           ("mixed", _lib.PREC_MIXED, torch.float32)):
+        s = PrimitiveEquationsSolver(4, prec)
+        s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+        plan = faces.FacePlan(s, b, lv, conn, kind=faces.LF_ADV, params=(0.3, -0.2, 0.1, 0.6))
+        U = torch.from_numpy(synth.to_lines(case.U, E, 5)).to("cuda").to(dt)
+        R = torch.empty_like(U)
+        s.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
+        ms_face = time_launches(lambda: plan.face_term(U, R, _lib.LAYOUT_LINES, stream), 20, 5,
+                                stream)
+        plan.attach()
+        ms = time_launches(lambda: s.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream), 20, 5,
+                           stream)
+        plan.detach()
+        out[f"headline_faces_{name}_face_term_us"] = round(ms_face * 1e3, 1)
+        out[f"headline_faces_{name}_rhs_with_faces_us"] = round(ms * 1e3, 1)
+        out[f"headline_faces_{name}_rhs_with_faces_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
+        plan.close()
+        s.close()
+        del U, R
+        torch.cuda.empty_cache()
+    out["headline_faces_mixed_vs_fp64_rhs_with_faces"] = round(
+        out["headline_faces_mixed_rhs_with_faces_gdof_s"] /
+        out["headline_faces_fp64_rhs_with_faces_gdof_s"], 3)
+    log(f"headline RHS with faces: {out}")
     return out
 
 
@@ -650,6 +734,9 @@ def main():
                          "instead of the autotuner's choice")
     ap.add_argument("--rhs-only", action="store_true",
                     help="time the FP64 RHS only (no mixed-precision or RK extras)")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="launch the mixed-precision RHS only, on the headline mesh (for "
+                         "instruction and traffic counters of that kernel)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time plain copies of known size instead (HBM counter calibration "
                          "and copy ceiling)")
@@ -676,6 +763,25 @@ def main():
     dof = E * nd
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream()
+
+    if args.profile_mixed:
+        s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
+        s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+        U32 = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev).to(torch.float32)
+        R32 = torch.empty_like(U32)
+        if args.launch:
+            wpc, remap = (int(v) for v in args.launch.split(","))
+            s32.set_launch(wpc, remap)
+        else:
+            s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
+        ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
+                             args.steps, args.warmup, stream)
+        print(json.dumps({"kernel": "volume_kernel<4, float, float, EPI_RHS>",
+                          "us": round(ms32 * 1e3, 2),
+                          "gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
+                          "launch": list(s32.get_launch())}), flush=True)
+        s32.close()
+        return
 
     s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
@@ -712,8 +818,10 @@ def main():
     s64.close()
     torch.cuda.empty_cache()
     if not args.rhs_only:
+        extras.update(measure_faces_uniform(args, case, stream))
         extras.update(measure_config2(args, stream))
         extras.update(measure_coastline(args, stream))
+        extras.update(measure_faces_coastline(args, stream))
         extras.update(measure_faces(stream))
         extras.update(measure_amr(stream))
         extras.update(measure_vertical(stream))
