@@ -7,9 +7,16 @@ src/solver/time_stepper.cpp:86-119) on about 400 sampled elements.
 Bars, per field (Hu, Hv, eta, HT, HS), relative L2 over the sample:
   FP64 right-hand side and one fused RK4 step    <= 1e-12
   mixed (FP32 volume) right-hand side            <= 2e-5
-  mixed state after 1 and after 100 RK4 steps    <= 1e-5
-The steps use the time step SimulationDriver::step would take from this state
-(CFL 0.5, dx_min = 2/(N+1), simulation_driver.cpp:40-42,108-131).
+  mixed state after 1 RK4 step                   <= 1e-5
+  mixed state after 100 RK4 steps of dt/20       <= 1e-5
+dt is the time step SimulationDriver::step would take from this state
+(CFL 0.5, dx_min = 2/(N+1), simulation_driver.cpp:40-42,108-131).  The
+element-local reference system amplifies perturbations of its state fast:
+an FP64 trajectory from the FP32 rounding of this initial state (2.5e-8)
+differs from the exact one by 1.1e-5 after 10 steps of dt and overflows to
+NaN within 100; after 100 steps of dt/20 the difference is 6.2e-7.  The
+100-step bar is therefore taken at dt/20, where the FP64 reference keeps its
+own input rounding below 1e-6 and the bar measures the FP32 kernels.
 """
 import numpy as np
 import pytest
@@ -105,18 +112,17 @@ def test_config4_mixed_rhs_and_100_rk4_steps(orc, c4):
     print(f"  mixed RHS rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) <= TOL_MIXED_RHS, errs
     del R32
-    ref = Usub
-    done = 0
-    for k in (1, 100):
-        for _ in range(k - done):
-            m.rk_step_dev(_lib.RK_RK4, 0.0, dt, U32, _lib.LAYOUT_LINES, fused=True)
-        ref = _oracle_steps(orc, ref, hsub, dt, k - done)
-        done = k
+    U0 = U32.clone()
+    for k, step in ((1, dt), (100, dt / 20.0)):
+        U32.copy_(U0)
+        for _ in range(k):
+            m.rk_step_dev(_lib.RK_RK4, 0.0, step, U32, _lib.LAYOUT_LINES, fused=True)
+        ref = _oracle_steps(orc, Usub, hsub, step, k)
         torch.cuda.synchronize()
         got = lines_rows(U32, idx, E, N)
         assert np.all(np.isfinite(got))
         errs = rel_l2_fields(got, ref, ND)
-        print(f"  mixed state after {k} RK4 steps (dt {dt:.4g}): rel L2 per field "
+        print(f"  mixed state after {k} RK4 steps (dt {step:.4g}): rel L2 per field "
               f"{['%.2e' % e for e in errs]}", flush=True)
         assert max(errs) <= TOL_MIXED_STATE, (k, errs)
     m.close()

@@ -5,7 +5,19 @@ RK4Classic, src/solver/time_stepper.cpp:86-119).
 
 Bars, per field, relative L2: right-hand side <= 2e-5 (FP32 rounding of the
 state, the operators and about 2(N+1) accumulated products per derivative);
-state after 1 and 100 RK4 steps <= 1e-5.  Every value must be finite.
+state after 1 RK4 step at the driver's CFL time step and after 100 RK4 steps
+at a twentieth of it <= 1e-5.  Every value must be finite.
+
+Why a twentieth for the 100 steps: the reference right-hand side is
+element-local and in reference coordinates (no face coupling, no metric
+terms), and its operators are strongly non-normal, so the system amplifies
+any perturbation of its state quickly.  At 8x8x4, N=3 the FP64 reference
+itself overflows to NaN within 100 steps at the CFL time step; an FP64
+trajectory started from the FP32 rounding of the initial state (a 2.5e-8
+perturbation) differs from the exact one by 1.2e-6 after 100 steps of
+CFL/10 and by 2.5e-7 after 100 steps of CFL/20.  CFL/20 is the horizon over
+which the FP64 reference keeps its own input rounding below 1e-6, so a
+1e-5 bar there measures the FP32 kernels, not the system's growth.
 """
 import numpy as np
 import pytest
@@ -26,7 +38,8 @@ def _solvers(case):
 
 
 def test_mixed_config1_rhs_and_steps(orc):
-    """8x8x4, N=3: every element; 1 and 100 RK4 steps at the driver's CFL dt."""
+    """8x8x4, N=3: every element; 1 RK4 step at the driver's CFL dt, 100 at
+    CFL/20."""
     import torch
 
     case = synth.config1()
@@ -46,18 +59,18 @@ def test_mixed_config1_rhs_and_steps(orc):
     H, u, v, _, _ = orc.derived(case.order, case.U, case.h)
     dt = AdaptiveTimeController(0.5, 1e-6, 3600.0).compute_dt(
         orc.max_wave_speed(case.order, H, u, v), 2.0 / n)
-    Uref = case.U.copy()
-    done = 0
-    for k in (1, 100):
-        for _ in range(k - done):
-            m.rk_step_dev(_lib.RK_RK4, 0.0, dt, U32, _lib.LAYOUT_LINES, fused=True)
-            Uref = orc.rk_step(orc.RK4, case.order, 0.0, dt, Uref, case.h, f)
-        done = k
+    U0 = U32.clone()
+    for k, step in ((1, dt), (100, dt / 20.0)):
+        U32.copy_(U0)
+        Uref = case.U.copy()
+        for _ in range(k):
+            m.rk_step_dev(_lib.RK_RK4, 0.0, step, U32, _lib.LAYOUT_LINES, fused=True)
+            Uref = orc.rk_step(orc.RK4, case.order, 0.0, step, Uref, case.h, f)
         torch.cuda.synchronize()
         got = synth.from_lines(U32.double().cpu().numpy(), E, n)
         assert np.all(np.isfinite(got))
         errs = rel_l2_fields(got, Uref, nd)
-        print(f"[mixed] after {k} RK4 steps (dt {dt:.4g}) rel L2 {['%.2e' % e for e in errs]}",
+        print(f"[mixed] after {k} RK4 steps (dt {step:.4g}) rel L2 {['%.2e' % e for e in errs]}",
               flush=True)
         assert max(errs) <= TOL_STATE, (k, errs)
     m.close()
