@@ -54,6 +54,7 @@ SIGNATURES = {
     "drf_max_wave_speed": (_I, [_P, _PD, _PD]),
     "drf_max_wave_speed_dev": (_I, [_P, _P, _I, _PD, _P]),
     "drf_max_wave_speed_derived_dev": (_I, [_P, _P, _P, _P, _L, _PD, _P]),
+    "drf_max_wave_speed_noalloc_dev": (_I, [_P, _P, _I, _P, _P]),
     "drf_compute_dt": (_D, [_D, _D, _D, _D, _D]),
     "drf_mode_split_step": (_I, [_P, _D, _I, _PD]),
     "drf_mode_split_step_dev": (_I, [_P, _D, _I, _P, _P]),

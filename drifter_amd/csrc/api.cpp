@@ -565,6 +565,11 @@ drf_status drf_create(int order, int precision, int device, drf_ctx **out)
     DRF_REQUIRE(device >= 0 && device < ndev, DRF_ERR_ARG,
                 "device " + std::to_string(device) + " not present");
     DRF_HIP_CHECK(hipSetDevice(device));
+    {
+        const drf_status st = register_scratch(device);
+        if (st)
+            return st;
+    }
     drf_ctx *c = new drf_ctx();
     c->order = order;
     c->n = order + 1;

@@ -138,6 +138,10 @@ hipError_t launch_adaptive_dt(const double *cmax, double dx_min, double cfl, dou
 // conforming and mortar contributions.
 drf_status face_term(const drf_face_plan *p, const void *U, void *dU, int layout, hipStream_t s);
 
+// Device address of the file-scope scratch array of noalloc.hip, looked up
+// once per device at context creation (the device must be current).
+drf_status register_scratch(int device);
+
 // Mode split (modesplit.hip)
 hipError_t launch_mode_split(int order, int64_t E, double dt3, int nsub, double *full,
                              View<const double> h, View<const double> f, double f0,

@@ -229,6 +229,17 @@ class PrimitiveEquationsSolver:
                                              _sptr(stream)))
         return c.value
 
+    def max_wave_speed_noalloc_dev(self, U, c_max, layout=_lib.LAYOUT_SOA, stream=None):
+        """The wave-speed maximum of U into the device float64 c_max (one
+        value), asynchronously and without allocating device memory (the
+        reduction's partials live in the library's per-GPU scratch array;
+        calls that overlap on one GPU race)."""
+        import torch
+
+        check(self._l.drf_max_wave_speed_noalloc_dev(
+            self._ctx, self._dev_state(U), layout, _dev_array(c_max, 1, torch.float64, "c_max"),
+            _sptr(stream)))
+
     def max_wave_speed_derived_dev(self, H, u, v, elem_stride, stream=None):
         import torch
 

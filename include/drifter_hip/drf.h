@@ -146,6 +146,16 @@ drf_status drf_max_wave_speed_dev(drf_ctx *ctx, const void *U, int layout, doubl
 drf_status drf_max_wave_speed_derived_dev(drf_ctx *ctx, const void *H, const void *u,
                                           const void *v, int64_t elem_stride, double *c_max,
                                           void *stream);
+/* No counterpart in the reference: the maximum of drf_max_wave_speed_dev,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  Allocates nothing: the
+ * reduction's partial maxima (at most 2,048 doubles) live in a file-scope
+ * device array of the library, one per GPU, shared by every call on that GPU
+ * (calls that overlap on one GPU race), and the maximum is written to the
+ * caller's device double c_max_dev, asynchronously on `stream`.  FP64
+ * contexts; bit-identical to drf_max_wave_speed_dev. */
+drf_status drf_max_wave_speed_noalloc_dev(drf_ctx *ctx, const void *U, int layout,
+                                          void *c_max_dev, void *stream);
 /* AdaptiveTimeController::compute_dt (src/solver/time_stepper.cpp:261-268). */
 double drf_compute_dt(double max_wave_speed, double dx_min, double cfl, double dt_min,
                       double dt_max);
