@@ -863,13 +863,17 @@ def main():
             f"{cpu['threaded']['value']:.4f} GDOF/s on {cpu['threaded']['cores']} threads "
             f"({cpu['cpu_model']})")
 
-    traffic, tsrc = None, None
+    # HBM bytes and FP64 FLOPs per launch of this kernel, from rocprofv3
+    # counter runs of the same command (profiles/volume_rhs_fp64_128x128x16_n4.md)
+    traffic, tsrc, flops = None, None, None
     tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_128x128x16_n4.traffic.json")
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
         traffic = tj.get("bytes_per_launch")
-        tsrc = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and launch shape, "
-                "from profiles/" + os.path.basename(tfile))
+        flops = tj.get("flops_per_launch")
+        tsrc = ("rocprofv3 --pmc FETCH_SIZE (x2.000) / WRITE_SIZE (x1.000) and "
+                "SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 of this kernel at launch shape "
+                f"{tj.get('launch')}, profiles/" + os.path.basename(tfile))
 
     line = {
         "metric": "DG RHS GDOF/s on 1 MI355X (FP64 vs mixed @ stated L2 err); % HBM roofline",
@@ -900,6 +904,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": tsrc,
+            "fp64_flops_per_launch": flops,
+            "fp64_tflops": round(flops / (ms * 1e-3) / 1e12, 2) if flops else None,
+            "fp64_tflops_peak": FP64_PEAK_TFLOPS,
+            "fp64_flop_frac": round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)
+                              if flops else None,
             "algorithmic_bytes_per_launch": bytes_per,
             "kernel": "volume_kernel<4, double, double, EPI_RHS>",
         },
