@@ -30,7 +30,9 @@
  * (tests/unit/test_amr.cpp, tests/integration/test_mesh.cpp: 2x2x2 = 8 and
  * boundary/interior mix of 2x2x1), brute-force geometric adjacency on uniform
  * meshes (every interior face's neighbour shares the face and points back),
- * 2:1 balance of every leaf pair sharing a face after refine, leaf order =
+ * 2:1 balance of every leaf pair sharing a face after refine, minimality of
+ * balance (every split it made has a node across a face whose level gap
+ * exceeds one on exactly the split's axes), leaf order =
  * depth-first order with children iz -> iy -> ix, and Morton codes against an
  * independent bit-interleaving in Python.
  */
@@ -383,6 +385,25 @@ void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t 
                 levels[3 * i + a] = N->lev[a];
         if (morton)
             morton[i] = N->morton;
+    }
+}
+
+int64_t orc_mesh_num_nodes(const orc_mesh *m) { return m->nn; }
+
+/* every node of the tree in creation order (ids are stable: nodes are only
+ * appended), leaves and interior nodes alike; mask is the split an interior
+ * node was given, leaf its element number or -1 */
+void orc_mesh_nodes(const orc_mesh *m, double *bounds, int *levels, int *mask, int *nchild,
+                    int64_t *leaf)
+{
+    for (int64_t i = 0; i < m->nn; ++i) {
+        const onode *N = &m->nodes[i];
+        memcpy(bounds + 6 * i, N->b, sizeof(N->b));
+        for (int a = 0; a < 3; ++a)
+            levels[3 * i + a] = N->lev[a];
+        mask[i] = N->mask;
+        nchild[i] = N->nchild;
+        leaf[i] = N->nchild ? -1 : N->leaf;
     }
 }
 

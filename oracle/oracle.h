@@ -112,6 +112,11 @@ void orc_mesh_balance(orc_mesh *m);
 int64_t orc_mesh_num_elements(const orc_mesh *m);
 /* bounds E*6, levels E*3, morton E (any may be NULL) */
 void orc_mesh_elements(const orc_mesh *m, double *bounds, int *levels, uint64_t *morton);
+/* node count and every node (bounds n*6, levels n*3, split mask, child count,
+ * element number or -1), in creation order */
+int64_t orc_mesh_num_nodes(const orc_mesh *m);
+void orc_mesh_nodes(const orc_mesh *m, double *bounds, int *levels, int *mask, int *nchild,
+                    int64_t *leaf);
 /* per (element, face): type, neighbour count, up to 4 neighbours with their
  * face ids and sub-face indices (unused slots -1) */
 void orc_mesh_connections(const orc_mesh *m, uint8_t *type, int8_t *count, int64_t *elems,

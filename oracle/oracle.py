@@ -65,6 +65,8 @@ def load():
         "orc_mesh_balance": (None, [C.c_void_p]),
         "orc_mesh_num_elements": (C.c_int64, [C.c_void_p]),
         "orc_mesh_elements": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "orc_mesh_num_nodes": (C.c_int64, [C.c_void_p]),
+        "orc_mesh_nodes": (None, [C.c_void_p] * 6),
         "orc_mesh_connections": (None, [C.c_void_p] * 6),
         "orc_mesh_connections_2to1": (None, [C.c_void_p] * 6),
         "orc_face_lists": (C.c_int, [C.c_int64] + [C.c_void_p] * 11),
@@ -250,6 +252,19 @@ class Mesh:
         mo = np.zeros(E, dtype=np.uint64)
         self._l.orc_mesh_elements(self.h, b.ctypes.data, lv.ctypes.data, mo.ctypes.data)
         return b, lv, mo
+
+    def nodes(self):
+        """Every tree node in creation order: bounds (n, 6), levels (n, 3),
+        split mask, child count, element number (-1 for interior nodes)."""
+        n = self._l.orc_mesh_num_nodes(self.h)
+        b = np.zeros((n, 6))
+        lv = np.zeros((n, 3), dtype=np.int32)
+        mk = np.zeros(n, dtype=np.int32)
+        nc = np.zeros(n, dtype=np.int32)
+        lf = np.zeros(n, dtype=np.int64)
+        self._l.orc_mesh_nodes(self.h, b.ctypes.data, lv.ctypes.data, mk.ctypes.data,
+                               nc.ctypes.data, lf.ctypes.data)
+        return b, lv, mk, nc, lf
 
     def connections(self, two_to_one=False):
         E = self.num_elements

@@ -200,6 +200,65 @@ def test_refined_connectivity_brute_force(orc, seed):
     assert nmortar > 0
 
 
+def _forcing_nodes(b, lv, x, mask, tol=1e-12):
+    """Nodes that would make balance split node x with `mask`: outside x, with
+    the probe point of one of their faces (the centre of the same-size box
+    across it) inside x's bounds (inclusive), and a level gap to x above one
+    on exactly the axes of mask (octree_adapter.cpp:138-182)."""
+    bx = b[x]
+    within = np.all((b[:, 0::2] >= bx[0::2] - tol) & (b[:, 1::2] <= bx[1::2] + tol), axis=1)
+    need = ((lv - lv[x]) > 1).astype(np.int64) @ np.array([1, 2, 4])
+    cand = (need == mask) & ~within
+    c = 0.5 * (b[:, 0::2] + b[:, 1::2])
+    s = b[:, 1::2] - b[:, 0::2]
+    found = []
+    for f in range(6):
+        a = f // 2
+        p = c.copy()
+        p[:, a] = b[:, 2 * a + 1] + 0.5 * s[:, a] if f % 2 else b[:, 2 * a] - 0.5 * s[:, a]
+        hit = cand & np.all((bx[0::2] <= p) & (p <= bx[1::2]), axis=1)
+        found.extend(np.nonzero(hit)[0].tolist())
+    return found
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 5])
+def test_balance_splits_only_what_a_level_gap_forces(orc, seed):
+    """Minimality of balance: every node that the balance pass split (not one
+    the caller refined) has a node across one of its faces that is more than
+    one level finer on exactly the axes of the split, and on no other axis.
+    The tree only gets finer, so the node that forced a split is still in the
+    final tree, found here by a linear scan over all nodes.  The same check
+    rejects most of the caller's own (unforced) splits, and a split with one
+    axis more than recorded, so it is not vacuous."""
+    rng = np.random.default_rng(100 + seed)
+    m = orc.Mesh(0.0, 1.0, 0.0, 2.0, -1.0, 0.0)
+    m.build_uniform(4, 4, 2)
+    nforced = nfree = nfree_unforced = nwider = 0
+    for _ in range(3):
+        _, _, _, nc0, lf0 = m.nodes()
+        sel = np.unique(rng.integers(0, m.num_elements, size=max(1, m.num_elements // 5)))
+        m.refine(sel, rng.integers(1, 8, size=sel.size))
+        b, lv, mk, nc, _ = m.nodes()
+        node_of_leaf = {int(l): i for i, l in enumerate(lf0) if l >= 0}
+        refined = {node_of_leaf[int(e)] for e in sel}
+        split_before = np.zeros(b.shape[0], bool)
+        split_before[:nc0.size] = nc0 > 0
+        for x in np.nonzero(nc > 0)[0]:
+            if split_before[x]:
+                continue
+            if x in refined:
+                nfree += 1
+                nfree_unforced += not _forcing_nodes(b, lv, x, mk[x])
+                continue
+            nforced += 1
+            assert _forcing_nodes(b, lv, x, mk[x]), (x, mk[x], lv[x])
+            if mk[x] != 7:
+                wider = mk[x] | (1 << int(np.nonzero([(mk[x] >> a) & 1 == 0 for a in range(3)])[0][0]))
+                nwider += not _forcing_nodes(b, lv, x, wider)
+    assert nforced > 0 and nwider > 0
+    assert nfree_unforced > nfree // 2
+
+
 def test_two_to_one_connectivity_a_coarse_face_lists_its_fine_leaves(orc):
     """The case the reference's descent cannot reach: a face whose same-size
     neighbour is refined lists the fine leaves across it (Fine2x2 here), and
