@@ -78,23 +78,8 @@ class CoastlineFields:
 def build_rows(fields, seed, estimator=None):
     """Features X (E, 128) BF16 bits, target y (E,) FP32 and the
     estimator's combined decision action (E,) uint8 for one seeded state."""
-    import torch
-
-    est = estimator or adapt.GradientErrorEstimator(ORDER)
     eta, T = fields.sample(seed)
-    E, nd = fields.E, fields.nd
-    X = torch.zeros((E, 2 * nd), dtype=torch.int16, device=eta.device)
-    vs = []
-    for col, f in ((0, eta), (nd, T)):
-        r = est.estimate(f, fields.h_char)
-        vs.append(r["value"])
-        inv = 1.0 / r["max_value"]  # device scalar: no host round trip
-        adapt.modal_features_bf16(ORDER, f, E, X, col, scale=1.0,
-                                  elem_scale=(fields.h_char * inv).contiguous())
-    v = torch.maximum(vs[0], vs[1]).clamp_min(FLOOR)
-    y = torch.log10(v).to(torch.float32)
-    action = mlp.classify(y, REFINE_LEVEL, COARSEN_LEVEL)
-    return X, y, action
+    return rows_from_fields(fields, eta, T, estimator)
 
 
 def build_dataset(fields, seeds):
@@ -133,3 +118,61 @@ def misclassification(model, X, action):
     pred = model.forward(X)
     got = mlp.classify(pred, REFINE_LEVEL, COARSEN_LEVEL)
     return float((got != action).float().mean().cpu())
+
+
+class SteppedCoastline:
+    """Coastline states that the model has stepped: the seeded Fourier state
+    (u, v, eta, T, S as in drifter_amd.synth, evaluated on the device) on the
+    coastline mesh with its bathymetry, advanced by fused RK4 steps of the
+    right-hand side at the CFL time step of a reference state."""
+
+    def __init__(self, fields, h_nodal, solver, dt):
+        self.fields, self.h, self.solver, self.dt = fields, h_nodal, solver, dt
+        self.n = ORDER + 1
+
+    def state(self, seed):
+        """The seed's state in the line layout [field][i][e][L], on the device."""
+        import torch
+
+        f = self.fields
+        u = 0.1 * f.fourier(0, seed)
+        v = 0.1 * f.fourier(1, seed)
+        eta = 0.05 * f.fourier(2, seed)
+        T = 15.0 + 5.0 * f.fourier(3, seed)
+        S = 32.5 + 2.5 * f.fourier(4, seed)
+        H = eta + self.h
+        n, E = self.n, f.E
+        P = torch.stack([H * u, H * v, eta, H * T, H * S], dim=1)  # (E, 5, nd)
+        return P.reshape(E, 5, n * n, n).permute(1, 3, 0, 2).contiguous().reshape(-1)
+
+    def eta_T(self, V):
+        n, E, nd = self.n, self.fields.E, self.fields.nd
+        P = V.view(5, n, E, n * n).permute(2, 0, 3, 1).reshape(E, 5, nd)
+        H = P[:, 2] + self.h
+        return P[:, 2].contiguous(), (P[:, 3] / H).contiguous()
+
+    def step(self, V, k):
+        from . import _lib
+
+        for _ in range(k):
+            self.solver.rk_step_dev(_lib.RK_RK4, 0.0, self.dt, V, _lib.LAYOUT_LINES, fused=True)
+
+
+def rows_from_fields(fields, eta, T, estimator=None):
+    """Features X (E, 128) BF16 bits, target y (E,) FP32 and the estimator's
+    combined decision (E,) uint8 of nodal (eta, T) fields (E, nd)."""
+    import torch
+
+    est = estimator or adapt.GradientErrorEstimator(ORDER)
+    E, nd = fields.E, fields.nd
+    X = torch.zeros((E, 2 * nd), dtype=torch.int16, device=eta.device)
+    vs = []
+    for col, f in ((0, eta), (nd, T)):
+        r = est.estimate(f, fields.h_char)
+        vs.append(r["value"])
+        inv = 1.0 / r["max_value"]  # device scalar: no host round trip
+        adapt.modal_features_bf16(ORDER, f, E, X, col, scale=1.0,
+                                  elem_scale=(fields.h_char * inv).contiguous())
+    v = torch.maximum(vs[0], vs[1]).clamp_min(FLOOR)
+    y = torch.log10(v).to(torch.float32)
+    return X, y, mlp.classify(y, REFINE_LEVEL, COARSEN_LEVEL)

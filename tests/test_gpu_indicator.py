@@ -37,3 +37,58 @@ def test_indicator_learns_estimator_decisions():
           f"trained {after:.4f} (train set {train_err:.4f}, train loss {loss:.4f})", flush=True)
     assert after < 0.5 * min(before, const)
     assert after < 0.15
+
+
+def test_indicator_does_not_anticipate_the_estimator():
+    """Trained on stepped runs to predict the estimator's decision K = 10 RK4
+    steps (CFL dt) ahead from the current modal coefficients, the network
+    does not beat reusing the current decision: its imitation error (about
+    10 %) is above how often the decision changes in 10 steps (about 3 %).
+    The test records that absence of a lookahead gain, and that the network
+    still learns the lookahead target far better than untrained."""
+    import torch
+
+    from drifter_amd import _lib, indicator, mlp, synth
+    from drifter_amd.mesh import coastline_mesh
+    from drifter_amd.solver import (AdaptiveTimeController, CoriolisParameter,
+                                    PrimitiveEquationsSolver)
+
+    mesh = coastline_mesh(band=7000.0)
+    fields = indicator.CoastlineFields(mesh)
+    E, nd = fields.E, fields.nd
+    bounds, _, _ = mesh.elements()
+    case = synth.make_case_on(bounds, 3, 100000.0, "coast")
+    s = PrimitiveEquationsSolver(3)
+    s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+    h = torch.from_numpy(case.h.reshape(E, nd)).cuda()
+    sc = indicator.SteppedCoastline(fields, h, s, 0.0)
+    V = sc.state(1000)
+    sc.dt = AdaptiveTimeController(0.5, 1e-6, 3600.0).compute_dt(
+        s.max_wave_speed_dev(V, _lib.LAYOUT_LINES), 2.0 / 4)
+    K = 10
+
+    def runs(seeds):
+        X0, A0, YK, AK = [], [], [], []
+        for sd in seeds:
+            V = sc.state(sd)
+            x, _, a = indicator.rows_from_fields(fields, *sc.eta_T(V))
+            sc.step(V, K)
+            _, y, ak = indicator.rows_from_fields(fields, *sc.eta_T(V))
+            X0.append(x)
+            A0.append(a)
+            YK.append(y)
+            AK.append(ak)
+        return torch.cat(X0), torch.cat(A0), torch.cat(YK), torch.cat(AK)
+
+    Xtr, _, ytr, _ = runs(range(1000, 1012))
+    Xte, a0, _, ak = runs(range(5000, 5002))
+    persistence = float((a0 != ak).float().mean())
+    model = mlp.IndicatorMLP(seed=7)
+    untrained = indicator.misclassification(model, Xte, ak)
+    indicator.train(model, Xtr, ytr, epochs=16, batch=2048, lr=2e-3)
+    net = indicator.misclassification(model, Xte, ak)
+    print(f"[indicator] K={K} lookahead: persistence {persistence:.4f}, network {net:.4f}, "
+          f"untrained {untrained:.4f}", flush=True)
+    assert net < 0.5 * untrained
+    assert persistence < net  # no lookahead gain: reported, not hidden
+    s.close()
