@@ -283,7 +283,7 @@ def measure_mixed_and_steps(args, case, s64, U, r64, stream, label):
     s32.close()
     return {
         "autotune_mixed": {"wg_per_cu": tune32[0], "xcd_remap": tune32[1],
-                           "us": round(tune32[2], 2)},
+                           "wg_lines": tune32[2], "us": round(tune32[3], 2)},
         "mixed_gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
         "mixed_ms": round(ms32, 6),
         "mixed_vs_fp64_gdof_ratio": None,
@@ -329,7 +329,7 @@ def measure_config2(args, stream):
     out["c2_fp64_gdof_s"] = round(dof * 5 / (ms * 1e-3) / 1e9, 3)
     out["c2_fp64_hbm_frac"] = round(s64.rhs_bytes() / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     out["c2_autotune_fp64"] = {"wg_per_cu": tune[0], "xcd_remap": tune[1],
-                               "us": round(tune[2], 2)}
+                               "wg_lines": tune[2], "us": round(tune[3], 2)}
     s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
@@ -691,6 +691,18 @@ def measure_vertical(stream):
     ms = time_launches(lambda: vv.diagnose_omega(U, h, stream), 20, 3, stream)
     out["diagnose_omega_32768_n4_us"] = round(ms * 1e3, 1)
     out["diagnose_omega_gb_s"] = round(n * 24 / (ms * 1e-3) / 1e9, 1)
+    # pressure gradient (density Jacobian and standard with the metric inputs)
+    from drifter_amd import pressure
+
+    rho = 1020.0 + 10.0 * r()
+    H = h + eta
+    for name, method, nbytes in (("jacobian", pressure.DENSITY_JACOBIAN, 6 * 8),
+                                 ("standard", pressure.STANDARD, 10 * 8)):
+        pg = pressure.PressureGradientSolver(4, method)
+        ms = time_launches(lambda: pg.compute(eta, rho, H, sg, dex, dey, dhx, dhy,
+                                              stream=stream), 20, 3, stream)
+        out[f"pressure_gradient_{name}_32768_n4_us"] = round(ms * 1e3, 1)
+        out[f"pressure_gradient_{name}_gb_s"] = round(n * nbytes / (ms * 1e-3) / 1e9, 1)
     log(f"sigma / vertical: {out}")
     return out
 
@@ -758,9 +770,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--launch", default=None, metavar="WPC,REMAP",
-                    help="fixed volume-kernel launch shape (workgroups per CU, XCD order 0/1) "
-                         "instead of the autotuner's choice")
+    ap.add_argument("--launch", default=None, metavar="WPC,REMAP[,LINES]",
+                    help="fixed volume-kernel launch shape (workgroups per CU, XCD order 0/1, "
+                         "grid lines per workgroup 128/256/512) instead of the autotuner's "
+                         "choice")
     ap.add_argument("--rhs-only", action="store_true",
                     help="time the FP64 RHS only (no mixed-precision or RK extras)")
     ap.add_argument("--profile-mixed", action="store_true",
@@ -799,8 +812,7 @@ def main():
         U32 = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev).to(torch.float32)
         R32 = torch.empty_like(U32)
         if args.launch:
-            wpc, remap = (int(v) for v in args.launch.split(","))
-            s32.set_launch(wpc, remap)
+            s32.set_launch(*(int(v) for v in args.launch.split(",")))
         else:
             s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
         ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
@@ -818,13 +830,13 @@ def main():
     R = torch.empty_like(U)
     # launch shape from the library's autotuner (workgroups per CU, XCD order)
     if args.launch:
-        wpc, remap = (int(v) for v in args.launch.split(","))
-        s64.set_launch(wpc, remap)
-        tune = (wpc, bool(remap), float("nan"))
+        s64.set_launch(*(int(v) for v in args.launch.split(",")))
+        tune = s64.get_launch() + (float("nan"),)
         log(f"launch shape fixed: {wpc} workgroups/CU, xcd_remap={bool(remap)}")
     else:
         tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
-        log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]:.1f} us")
+        log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]} lines per "
+            f"workgroup, {tune[3]:.1f} us")
     # One input/output pair: each launch reads 1.44 GB and writes 1.31 GB, so
     # the 256 MB last-level cache holds under a tenth of it between launches
     # and the state is read from HBM (at 64x64x8 the timed launches had to
@@ -837,7 +849,8 @@ def main():
     log(f"FP64 RHS ({label}): {ms * 1e3:.1f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s")
 
     extras = {"autotune_fp64": {"wg_per_cu": tune[0], "xcd_remap": tune[1],
-                                "us": round(tune[2], 2) if tune[2] == tune[2] else None}}
+                                "wg_lines": tune[2],
+                                "us": round(tune[3], 2) if tune[3] == tune[3] else None}}
     if not args.rhs_only:
         r64 = R.cpu().numpy()
         extras.update(measure_mixed_and_steps(args, case, s64, U, r64, stream, label))

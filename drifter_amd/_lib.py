@@ -46,8 +46,8 @@ SIGNATURES = {
     "drf_initialize": (_I, [_P, _L, _PD, _PD, _PD, _D, _D, _D, _PD]),
     "drf_rhs": (_I, [_P, _D, _PD, _PD]),
     "drf_rhs_dev": (_I, [_P, _D, _P, _P, _I, _P]),
-    "drf_set_launch": (_I, [_P, _I, _I]),
-    "drf_get_launch": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "drf_set_launch": (_I, [_P, _I, _I, _I]),
+    "drf_get_launch": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "drf_autotune": (_I, [_P, _P, _P, _I, _I, _P, C.POINTER(C.c_float)]),
     "drf_rk_step": (_I, [_P, _I, _D, _D, _PD]),
     "drf_rk_step_dev": (_I, [_P, _I, _D, _D, _P, _I, _I, _P]),

@@ -154,7 +154,7 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.dtp = dtp;
         epi.dtq = dtq;
         VolGeom<double> geo{c->E, coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
-                            c->f0, c->wg_per_cu, c->xcd_remap};
+                            c->f0, c->wg_per_cu, c->xcd_remap, c->wg_lines};
         err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
                                  s);
     } else {
@@ -173,7 +173,7 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.dtp = dtp;
         epi.dtq = dtq;
         VolGeom<float> geo{c->E, coef_view<float>(c->d_h32, c), coef_view<float>(c->d_f32, c),
-                           c->f0, c->wg_per_cu, c->xcd_remap};
+                           c->f0, c->wg_per_cu, c->xcd_remap, c->wg_lines};
         err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
                                  s);
     }
@@ -681,20 +681,24 @@ drf_status drf_rhs_dev(drf_ctx *c, double t, const void *U, void *dUdt, int layo
     return rhs_into(c, U, dUdt, layout, pick_stream(c, stream));
 }
 
-drf_status drf_set_launch(drf_ctx *c, int wg_per_cu, int xcd_remap)
+drf_status drf_set_launch(drf_ctx *c, int wg_per_cu, int xcd_remap, int wg_lines)
 {
     DRF_REQUIRE(c, DRF_ERR_ARG, "null context");
     DRF_REQUIRE(wg_per_cu >= 0 && wg_per_cu <= 64, DRF_ERR_ARG, "wg_per_cu must be 0..64");
+    DRF_REQUIRE(wg_lines == 0 || wg_lines == 128 || wg_lines == 256 || wg_lines == 512,
+                DRF_ERR_ARG, "wg_lines must be 0, 128, 256 or 512");
     c->wg_per_cu = wg_per_cu;
     c->xcd_remap = xcd_remap ? 1 : 0;
+    c->wg_lines = wg_lines ? wg_lines : 128;
     return DRF_OK;
 }
 
-drf_status drf_get_launch(const drf_ctx *c, int *wg_per_cu, int *xcd_remap)
+drf_status drf_get_launch(const drf_ctx *c, int *wg_per_cu, int *xcd_remap, int *wg_lines)
 {
-    DRF_REQUIRE(c && wg_per_cu && xcd_remap, DRF_ERR_ARG, "null argument");
+    DRF_REQUIRE(c && wg_per_cu && xcd_remap && wg_lines, DRF_ERR_ARG, "null argument");
     *wg_per_cu = c->wg_per_cu;
     *xcd_remap = c->xcd_remap;
+    *wg_lines = c->wg_lines;
     return DRF_OK;
 }
 
@@ -742,13 +746,20 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
             break;
         }
     }
-    // resident workgroups per CU (values above the occupancy limit clamp to
-    // it) x XCD-aware batch order
+    // workgroup size x resident workgroups per CU (values above the occupancy
+    // limit clamp to it; wider workgroups hold fewer per CU) x XCD-aware
+    // batch order.  Orders above 4 have the 128-line workgroup only.
     static const int kWpc[] = {1, 2, 3, 4, 5, 6, 8};
+    static const int kLines[] = {128, 256, 512};
+    const int nlines = c->order <= 4 ? 3 : 1;
     float best = 1e30f;
-    int bw = c->wg_per_cu, br = c->xcd_remap;
+    int bw = c->wg_per_cu, br = c->xcd_remap, bl = c->wg_lines;
+    for (int li = 0; li < nlines && res == DRF_OK; ++li)
     for (int remap = 0; remap < 2 && res == DRF_OK; ++remap)
         for (int wpc : kWpc) {
+            if (kLines[li] * wpc > 2048) // beyond a CU's 32 waves
+                continue;
+            c->wg_lines = kLines[li];
             c->wg_per_cu = wpc;
             c->xcd_remap = remap;
             for (int k = 0; k < nset && res == DRF_OK; ++k) // warm-up
@@ -769,6 +780,7 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
                 best = us;
                 bw = wpc;
                 br = remap;
+                bl = kLines[li];
             }
         }
     (void)hipStreamSynchronize(s);
@@ -780,6 +792,7 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
     (void)hipEventDestroy(e1);
     c->wg_per_cu = bw;
     c->xcd_remap = br;
+    c->wg_lines = bl;
     if (best_us)
         *best_us = best;
     return res;

@@ -111,16 +111,52 @@ template <class T> struct VolGeom {
     double f0;
     int wg_per_cu;   // launch shape: resident workgroups per CU (0: occupancy limit)
     int xcd_remap;   // renumber workgroups so neighbouring batches share an XCD
+    int wg_lines;    // grid lines per workgroup: 128, 256 or 512 (orders above 4: 128)
 };
 
-// Launchers (volume.hip).  Return hipError_t of the launch.
-// ops is always given in FP64; the FP32 launcher rounds it once on the host.
-hipError_t launch_volume_fp64(int order, const VolGeom<double> &geo, const Ops &ops,
-                              const Phys &ph, View<const double> U, const Epi<double> &epi,
-                              hipStream_t s);
-hipError_t launch_volume_fp32(int order, const VolGeom<float> &geo, const Ops &ops,
-                              const Phys &ph, View<const float> U, const Epi<float> &epi,
-                              hipStream_t s);
+// Launchers, one translation unit per workgroup size (volume_fp64*.hip,
+// volume_fp32*.hip).  Return hipError_t of the launch.  ops is always given
+// in FP64; the FP32 launchers round it once on the host.
+hipError_t launch_volume_fp64_w128(int order, const VolGeom<double> &geo, const Ops &ops,
+                                   const Phys &ph, View<const double> U, const Epi<double> &epi,
+                                   hipStream_t s);
+hipError_t launch_volume_fp64_w256(int order, const VolGeom<double> &geo, const Ops &ops,
+                                   const Phys &ph, View<const double> U, const Epi<double> &epi,
+                                   hipStream_t s);
+hipError_t launch_volume_fp64_w512(int order, const VolGeom<double> &geo, const Ops &ops,
+                                   const Phys &ph, View<const double> U, const Epi<double> &epi,
+                                   hipStream_t s);
+hipError_t launch_volume_fp32_w128(int order, const VolGeom<float> &geo, const Ops &ops,
+                                   const Phys &ph, View<const float> U, const Epi<float> &epi,
+                                   hipStream_t s);
+hipError_t launch_volume_fp32_w256(int order, const VolGeom<float> &geo, const Ops &ops,
+                                   const Phys &ph, View<const float> U, const Epi<float> &epi,
+                                   hipStream_t s);
+hipError_t launch_volume_fp32_w512(int order, const VolGeom<float> &geo, const Ops &ops,
+                                   const Phys &ph, View<const float> U, const Epi<float> &epi,
+                                   hipStream_t s);
+
+inline hipError_t launch_volume_fp64(int order, const VolGeom<double> &geo, const Ops &ops,
+                                   const Phys &ph, View<const double> U, const Epi<double> &epi,
+                                   hipStream_t s)
+{
+    switch (order <= 4 ? geo.wg_lines : 128) {
+    case 256: return launch_volume_fp64_w256(order, geo, ops, ph, U, epi, s);
+    case 512: return launch_volume_fp64_w512(order, geo, ops, ph, U, epi, s);
+    default: return launch_volume_fp64_w128(order, geo, ops, ph, U, epi, s);
+    }
+}
+
+inline hipError_t launch_volume_fp32(int order, const VolGeom<float> &geo, const Ops &ops,
+                                   const Phys &ph, View<const float> U, const Epi<float> &epi,
+                                   hipStream_t s)
+{
+    switch (order <= 4 ? geo.wg_lines : 128) {
+    case 256: return launch_volume_fp32_w256(order, geo, ops, ph, U, epi, s);
+    case 512: return launch_volume_fp32_w512(order, geo, ops, ph, U, epi, s);
+    default: return launch_volume_fp32_w128(order, geo, ops, ph, U, epi, s);
+    }
+}
 
 // Vector kernels (stepper.hip)
 hipError_t launch_wave_speed(int order, int64_t E, View<const double> U, View<const double> h,
@@ -172,6 +208,7 @@ struct drf_ctx {
     // volume-kernel launch shape (drf_set_launch / drf_autotune)
     int wg_per_cu = 0;
     int xcd_remap = 1;
+    int wg_lines = 128;
     // face term added to every right-hand side (drf_set_face_plan), or null
     drf_face_plan *faces = nullptr;
 };

@@ -1,15 +1,16 @@
-// Volume right-hand side kernels, FP32 storage and arithmetic (mixed-precision path; the free-surface column integrals stay FP64).
-// The kernel itself is in volume_impl.hpp; this unit instantiates it for
-// orders 1-8 and every fused epilogue.
+// Volume right-hand side kernels, FP32 storage and arithmetic (mixed-precision
+// path; the free-surface column integrals stay FP64),
+// 128 grid lines per workgroup.  The kernel itself is in volume_impl.hpp; this
+// unit instantiates it for orders 1-8 and every fused epilogue.
 #include "volume_impl.hpp"
 
 namespace drf {
 
-hipError_t launch_volume_fp32(int order, const VolGeom<float> &geo, const Ops &ops,
-                              const Phys &ph, View<const float> U, const Epi<float> &epi,
-                              hipStream_t s)
+hipError_t launch_volume_fp32_w128(int order, const VolGeom<float> &geo, const Ops &ops,
+                                   const Phys &ph, View<const float> U, const Epi<float> &epi,
+                                   hipStream_t s)
 {
-    return vol::dispatch<float, float>(order, geo, ops, ph, U, epi, s);
+    return vol::dispatch<float, float, 128>(order, geo, ops, ph, U, epi, s);
 }
 
 } // namespace drf

@@ -40,11 +40,17 @@ namespace vol {
 // LDS slots per element node, used in two rounds (see the kernel).
 enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_G, S_NF };
 
-template <int N> struct Cfg {
+// LDS a workgroup may take for its elements (gfx950: 160 KB per workgroup),
+// leaving room for the staged D.
+constexpr int kLdsBudget = 160 * 1024 - 1024;
+
+// LW: grid lines per workgroup (128, 256 or 512), the workgroup size; the
+// elements per batch EPB follow from it and from the LDS budget.
+template <int N, int LW> struct Cfg {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
     static constexpr int lds_per_elem_fp64 = (S_NF * n3 + 4 * n2) * 8; // slots + column integrals
-    static constexpr int epb_thr = (128 / n2) > 0 ? 128 / n2 : 1;
-    static constexpr int epb_lds = (65536 / lds_per_elem_fp64) > 0 ? 65536 / lds_per_elem_fp64 : 1;
+    static constexpr int epb_thr = (LW / n2) > 0 ? LW / n2 : 1;
+    static constexpr int epb_lds = (kLdsBudget / lds_per_elem_fp64) > 0 ? kLdsBudget / lds_per_elem_fp64 : 1;
     static constexpr int EPB = epb_thr < epb_lds ? epb_thr : epb_lds;
     static constexpr int NT = EPB * n2;
     static constexpr int BLOCK = ((NT + 63) / 64) * 64;
@@ -132,16 +138,21 @@ __device__ __forceinline__ void d2line(const TC *__restrict__ D, const TC *__res
     }
 }
 
-// The state lines a thread needs for phase P1.
+// The state lines a thread needs for phase P1, with the Coriolis parameter
+// along the line (a field, or the constant f0).
 template <int n, class TC> struct Lines {
-    TC hu[n], hv[n], et[n], ht[n], hs[n], hb[n];
+    TC hu[n], hv[n], et[n], ht[n], hs[n], hb[n], fc[n];
 };
 
+// All of P1's global loads are issued here, so that P1 itself waits only for
+// them: a load issued inside P1 would make the wave wait for every store of
+// the previous batch as well (gfx9 counts loads and stores in one in-order
+// counter).
 template <int n, class TS, class TC>
 __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS> &U,
-                                           const View<const TS> &h, int64_t e, int line)
+                                           const VolGeom<TS> &geo, int64_t e, int line)
 {
-    const uint32_t uo = lane_off(U, e, line), ho = lane_off(h, e, line);
+    const uint32_t uo = lane_off(U, e, line), ho = lane_off(geo.h, e, line);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
         L.hu[i] = ldo<TS, TC>(U.p + 0 * U.fs + i * U.is, uo);
@@ -149,7 +160,17 @@ __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS>
         L.et[i] = ldo<TS, TC>(U.p + 2 * U.fs + i * U.is, uo);
         L.ht[i] = ldo<TS, TC>(U.p + 3 * U.fs + i * U.is, uo);
         L.hs[i] = ldo<TS, TC>(U.p + 4 * U.fs + i * U.is, uo);
-        L.hb[i] = ldo<TS, TC>(h.p + i * h.is, ho);
+        L.hb[i] = ldo<TS, TC>(geo.h.p + i * geo.h.is, ho);
+    }
+    if (geo.f.p) {
+        const uint32_t fo = lane_off(geo.f, e, line);
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            L.fc[i] = ldo<TS, TC>(geo.f.p + i * geo.f.is, fo);
+    } else {
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            L.fc[i] = (TC)geo.f0;
     }
 }
 
@@ -238,12 +259,12 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
 //   P23b eta-owner eta-derivatives of the tracer fluxes    -> A, B
 //        zeta-owner vertical diffusion of T, S (same phase) -> C, D
 //   P4b xi-owner   dHT, dHS and d eta complete (written out)
-template <int N, class TS, class TC, int EPI>
-__global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(Cfg<N>::template wpe<TS, EPI>())))
+template <int N, class TS, class TC, int EPI, int LW>
+__global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_waves_per_eu(Cfg<N, LW>::template wpe<TS, EPI>())))
     volume_kernel(const VolGeom<TS> geo, const KOps<N + 1, TC> ops, const Phys ph,
                   const View<const TS> U, const Epi<TS> epi, const int64_t nbatch)
 {
-    using C = Cfg<N>;
+    using C = Cfg<N, LW>;
     // Fused stages load their operand lines in the epilogue; the next-batch
     // prefetch is kept for the plain RHS, where registers allow it.
     constexpr bool PREFETCH = EPI == EPI_RHS;
@@ -252,11 +273,30 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
     // column integrals, double-buffered by batch parity: the last phase of
     // one batch reads them while the first phase of the next writes
     __shared__ double bar[2][EPB][2][n2];
+    // the FP64 D of the column integrals, indexed per lane in P4b (from the
+    // kernel arguments such an index compiles to a global load, and its wait
+    // would drain the next batch's prefetch and this batch's stores)
+    __shared__ double sDc[n * n];
 
+    // The plain RHS runs every lane through every phase, so that no phase
+    // sits behind an exec-mask branch: lanes past the tile repeat the last
+    // line of the last element (in the same wavefront, in lockstep), and
+    // elements past the end of the mesh repeat element E - 1 in their own LDS
+    // slots.  Their stores write the values the owners write.  With the
+    // stores on every path, P1 waits only for the prefetched lines and not
+    // for the previous batch's stores to drain.  The fused stages, whose
+    // epilogues read and rewrite their accumulators, mask such lanes.
+    constexpr bool DUP = PREFETCH;
     const int tid = threadIdx.x;
-    const bool in_tile = tid < C::NT;
-    const int le = in_tile ? tid / n2 : 0;
-    const int L = in_tile ? tid - le * n2 : 0;
+    const bool in_tile = DUP || tid < C::NT;
+    const int tq = (DUP && tid >= C::NT) ? C::NT - 1 : tid;
+    const int le = in_tile ? tq / n2 : 0;
+    const int L = in_tile ? tq - le * n2 : 0;
+    const int64_t elast = geo.E - 1;
+    auto elem = [&](int64_t bb) {
+        const int64_t x = bb * EPB + le;
+        return (DUP && x > elast) ? elast : x;
+    };
 
     const TC g = (TC)ph.g, nu_h = (TC)ph.nu_h, nu_v = (TC)ph.nu_v, kT = (TC)ph.kT, kS = (TC)ph.kS;
     const TC *__restrict__ D = ops.D;
@@ -277,21 +317,25 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                            ? (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
                            : (int64_t)blockIdx.x;
     int64_t b = lw;
+    if (tid < n * n)
+        sDc[tid] = Dc[tid]; // first read in P4b, after P1's barrier
     Lines<n, TC> nxt;
     if (PREFETCH) {
-        const int64_t e0 = b * EPB + le;
+        const int64_t e0 = elem(b);
         if (in_tile && b < nbatch && e0 < geo.E)
-            load_lines<n, TS, TC>(nxt, U, geo.h, e0, L);
+            load_lines<n, TS, TC>(nxt, U, geo, e0, L);
     }
 
     for (int par = 0; b < nbatch; b += G, par ^= 1) {
-        const int64_t e = b * EPB + le;
-        const bool act = in_tile && e < geo.E;
-        Lines<n, TC> cur;
-        if (PREFETCH)
-            cur = nxt;
-        else if (act)
-            load_lines<n, TS, TC>(cur, U, geo.h, e, L);
+        const int64_t e = elem(b);
+        const bool act = DUP || (in_tile && e < geo.E);
+        // With the prefetch, P1 reads the prefetched lines in place (the
+        // next batch's loads overwrite them only after P1's barrier), so the
+        // loop carries no register copies that would wait for those loads.
+        Lines<n, TC> own;
+        if (!PREFETCH && act)
+            load_lines<n, TS, TC>(own, U, geo, e, L);
+        const Lines<n, TC> &cur = PREFETCH ? nxt : own;
         TC *s = &sm[le][0][0];
 
         // Register state carried by the xi-owner: the momentum partials
@@ -311,17 +355,6 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
                 ht[i] = cur.ht[i];
                 hs[i] = cur.hs[i];
             }
-            TC fc[n];
-            if (geo.f.p) {
-                const uint32_t fo = lane_off(geo.f, e, L);
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                    fc[i] = ldo<TS, TC>(geo.f.p + i * geo.f.is, fo);
-            } else {
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                    fc[i] = (TC)geo.f0;
-            }
             // xi-derivatives (momentum_advection :275-305,
             // barotropic_pressure_gradient :322-337, horizontal_diffusion
             // :339-355) and Coriolis (:307-320)
@@ -332,14 +365,14 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             dline<n>(D, x, d);
 #pragma unroll
             for (int i = 0; i < n; ++i)
-                pHu[i] = -d[i] + fc[i] * cur.hv[i];
+                pHu[i] = -d[i] + cur.fc[i] * cur.hv[i];
 #pragma unroll
             for (int i = 0; i < n; ++i)
                 x[i] = cur.hv[i] * u[i];
             dline<n>(D, x, d);
 #pragma unroll
             for (int i = 0; i < n; ++i)
-                pHv[i] = -d[i] + (-fc[i]) * cur.hu[i];
+                pHv[i] = -d[i] + (-cur.fc[i]) * cur.hu[i];
             dline<n>(D, cur.et, d);
 #pragma unroll
             for (int i = 0; i < n; ++i)
@@ -379,9 +412,9 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
         if (PREFETCH) {
             // Next batch of this workgroup, in flight during the other phases.
             const int64_t bn = b + G;
-            const int64_t en = bn * EPB + le;
+            const int64_t en = elem(bn);
             if (in_tile && bn < nbatch && en < geo.E)
-                load_lines<n, TS, TC>(nxt, U, geo.h, en, L);
+                load_lines<n, TS, TC>(nxt, U, geo, en, L);
         }
 
         // ---------------- P2a: eta-line (i, k) = (L % n, L / n) ----------------
@@ -547,8 +580,8 @@ __global__ void __launch_bounds__(Cfg<N>::BLOCK) __attribute__((amdgpu_waves_per
             double s1 = 0.0, s2 = 0.0;
 #pragma unroll
             for (int m = 0; m < n; ++m) {
-                s1 += Dc[j0 * n + m] * bar[par][le][0][m + n * k0];
-                s2 += Dc[k0 * n + m] * bar[par][le][1][j0 + n * m];
+                s1 += sDc[j0 * n + m] * bar[par][le][0][m + n * k0];
+                s2 += sDc[k0 * n + m] * bar[par][le][1][j0 + n * m];
             }
             const TC deta = (TC)(-(s1 + s2));
             TC kv[3][n];
@@ -598,15 +631,15 @@ inline int num_cus()
     return cus;
 }
 
-template <int N, class TS, class TC, int EPI>
+template <int N, class TS, class TC, int EPI, int LW>
 hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
                      const Epi<TS> &epi, hipStream_t s)
 {
-    using C = Cfg<N>;
+    using C = Cfg<N, LW>;
     if (geo.E <= 0)
         return hipSuccess;
     const int64_t nbatch = (geo.E + C::EPB - 1) / C::EPB;
-    auto kern = volume_kernel<N, TS, TC, EPI>;
+    auto kern = volume_kernel<N, TS, TC, EPI, LW>;
     // resident workgroups per CU (queried once per instantiation: the launch
     // path stays free of runtime queries)
     static const int per_cu = [&] {
@@ -627,36 +660,44 @@ hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, Vie
     return hipGetLastError();
 }
 
-template <int N, class TS, class TC>
+template <int N, class TS, class TC, int LW>
 hipError_t launch_n(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
                     const Epi<TS> &epi, hipStream_t s)
 {
     switch (epi.mode) {
-    case EPI_RHS: return launch_ne<N, TS, TC, EPI_RHS>(geo, ops, ph, U, epi, s);
-    case EPI_STAGE: return launch_ne<N, TS, TC, EPI_STAGE>(geo, ops, ph, U, epi, s);
-    case EPI_STAGE_ACC: return launch_ne<N, TS, TC, EPI_STAGE_ACC>(geo, ops, ph, U, epi, s);
-    case EPI_RK4FIN: return launch_ne<N, TS, TC, EPI_RK4FIN>(geo, ops, ph, U, epi, s);
-    case EPI_COMB: return launch_ne<N, TS, TC, EPI_COMB>(geo, ops, ph, U, epi, s);
-    case EPI_LSRK: return launch_ne<N, TS, TC, EPI_LSRK>(geo, ops, ph, U, epi, s);
+    case EPI_RHS: return launch_ne<N, TS, TC, EPI_RHS, LW>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE: return launch_ne<N, TS, TC, EPI_STAGE, LW>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE_ACC: return launch_ne<N, TS, TC, EPI_STAGE_ACC, LW>(geo, ops, ph, U, epi, s);
+    case EPI_RK4FIN: return launch_ne<N, TS, TC, EPI_RK4FIN, LW>(geo, ops, ph, U, epi, s);
+    case EPI_COMB: return launch_ne<N, TS, TC, EPI_COMB, LW>(geo, ops, ph, U, epi, s);
+    case EPI_LSRK: return launch_ne<N, TS, TC, EPI_LSRK, LW>(geo, ops, ph, U, epi, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <class TS, class TC>
+// Orders above 4 run at 128 lines per workgroup only: their registers exceed
+// what wider workgroups allow (256 VGPRs at two waves per SIMD).
+template <class TS, class TC, int LW>
 hipError_t dispatch(int order, const VolGeom<TS> &geo, const Ops &ops, const Phys &ph,
                     View<const TS> U, const Epi<TS> &epi, hipStream_t s)
 {
     switch (order) {
-    case 1: return launch_n<1, TS, TC>(geo, ops, ph, U, epi, s);
-    case 2: return launch_n<2, TS, TC>(geo, ops, ph, U, epi, s);
-    case 3: return launch_n<3, TS, TC>(geo, ops, ph, U, epi, s);
-    case 4: return launch_n<4, TS, TC>(geo, ops, ph, U, epi, s);
-    case 5: return launch_n<5, TS, TC>(geo, ops, ph, U, epi, s);
-    case 6: return launch_n<6, TS, TC>(geo, ops, ph, U, epi, s);
-    case 7: return launch_n<7, TS, TC>(geo, ops, ph, U, epi, s);
-    case 8: return launch_n<8, TS, TC>(geo, ops, ph, U, epi, s);
-    default: return hipErrorInvalidValue;
+    case 1: return launch_n<1, TS, TC, LW>(geo, ops, ph, U, epi, s);
+    case 2: return launch_n<2, TS, TC, LW>(geo, ops, ph, U, epi, s);
+    case 3: return launch_n<3, TS, TC, LW>(geo, ops, ph, U, epi, s);
+    case 4: return launch_n<4, TS, TC, LW>(geo, ops, ph, U, epi, s);
+    default: break;
     }
+    if constexpr (LW == 128) {
+        switch (order) {
+        case 5: return launch_n<5, TS, TC, LW>(geo, ops, ph, U, epi, s);
+        case 6: return launch_n<6, TS, TC, LW>(geo, ops, ph, U, epi, s);
+        case 7: return launch_n<7, TS, TC, LW>(geo, ops, ph, U, epi, s);
+        case 8: return launch_n<8, TS, TC, LW>(geo, ops, ph, U, epi, s);
+        default: break;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 } // namespace vol

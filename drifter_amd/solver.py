@@ -156,23 +156,27 @@ class PrimitiveEquationsSolver:
             dUdt[...] = self.compute_rhs(U, t)
         return f
 
-    def set_launch(self, wg_per_cu=0, xcd_remap=True):
-        """Volume-kernel launch shape (0 workgroups per CU = occupancy limit)."""
-        check(self._l.drf_set_launch(self._ctx, int(wg_per_cu), int(bool(xcd_remap))))
+    def set_launch(self, wg_per_cu=0, xcd_remap=True, wg_lines=0):
+        """Volume-kernel launch shape: resident workgroups per CU (0 = the
+        occupancy limit), XCD-aware batch order, grid lines per workgroup
+        (0 = 128, 256, 512; orders above 4 run at 128)."""
+        check(self._l.drf_set_launch(self._ctx, int(wg_per_cu), int(bool(xcd_remap)),
+                                     int(wg_lines)))
 
     def get_launch(self):
-        w, r = C.c_int(), C.c_int()
-        check(self._l.drf_get_launch(self._ctx, C.byref(w), C.byref(r)))
-        return w.value, bool(r.value)
+        """(wg_per_cu, xcd_remap, wg_lines)"""
+        w, r, n = C.c_int(), C.c_int(), C.c_int()
+        check(self._l.drf_get_launch(self._ctx, C.byref(w), C.byref(r), C.byref(n)))
+        return w.value, bool(r.value), n.value
 
     def autotune(self, U, R, layout=_lib.LAYOUT_LINES, reps=10, stream=None):
         """Time every launch shape on (U, R), keep the fastest; returns
-        (wg_per_cu, xcd_remap, microseconds per launch)."""
+        (wg_per_cu, xcd_remap, wg_lines, microseconds per launch)."""
         us = C.c_float()
         check(self._l.drf_autotune(self._ctx, self._dev_state(U), self._dev_state(R, "dUdt"),
                                    layout, reps, _sptr(stream), C.byref(us)))
-        w, r = self.get_launch()
-        return w, r, us.value
+        w, r, n = self.get_launch()
+        return w, r, n, us.value
 
     def compute_rhs_dev(self, U, dUdt, layout=_lib.LAYOUT_SOA, stream=None, t=0.0):
         check(self._l.drf_rhs_dev(self._ctx, t, self._dev_state(U), self._dev_state(dUdt, "dUdt"),

@@ -112,16 +112,19 @@ drf_status drf_rhs_dev(drf_ctx *ctx, double t, const void *U, void *dUdt, int la
                        void *stream);
 
 /* Launch shape of the volume kernel (no counterpart in the reference):
- * resident workgroups per CU (0 = the occupancy limit, the default) and
- * the XCD-aware batch order (1, the default).  drf_autotune times
+ * resident workgroups per CU (0 = the occupancy limit, the default), the
+ * XCD-aware batch order (1, the default) and the workgroup size in grid lines
+ * (128, the default, 256 or 512; 0 = 128; orders above 4 run at 128 whatever
+ * is set: a wider workgroup stages more elements per batch and reads longer
+ * contiguous runs of each field plane).  drf_autotune times
  * drf_rhs_dev on U for each shape (reps launches each, on `stream`), keeps
  * the fastest in the context and returns its time per launch.  The timed
  * launches cycle over U / dUdt and three temporary device copies of them, so
  * that the state comes from HBM rather than from the last-level cache; the
  * copies are allocated and freed by the call (on the caller's buffers alone
  * if they cannot be allocated; not for use under stream capture). */
-drf_status drf_set_launch(drf_ctx *ctx, int wg_per_cu, int xcd_remap);
-drf_status drf_get_launch(const drf_ctx *ctx, int *wg_per_cu, int *xcd_remap);
+drf_status drf_set_launch(drf_ctx *ctx, int wg_per_cu, int xcd_remap, int wg_lines);
+drf_status drf_get_launch(const drf_ctx *ctx, int *wg_per_cu, int *xcd_remap, int *wg_lines);
 drf_status drf_autotune(drf_ctx *ctx, const void *U, void *dUdt, int layout, int reps,
                         void *stream, float *best_us);
 

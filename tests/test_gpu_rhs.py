@@ -127,16 +127,17 @@ def test_launch_shape_does_not_change_results(prec):
     s.set_launch(0, True)
     s.compute_rhs_dev(U, ref, _lib.LAYOUT_LINES)
     out = torch.empty_like(U)
-    for wpc in (1, 2, 3, 8):
-        for remap in (False, True):
-            s.set_launch(wpc, remap)
-            out.zero_()
-            s.compute_rhs_dev(U, out, _lib.LAYOUT_LINES)
-            torch.cuda.synchronize()
-            assert torch.equal(out, ref), (wpc, remap)
-    wpc, remap, us = s.autotune(U, out, _lib.LAYOUT_LINES, reps=2)
-    assert 1 <= wpc <= 8 and us > 0
-    assert s.get_launch() == (wpc, remap)
+    for lines in (128, 256, 512):
+        for wpc in (1, 2, 3, 8):
+            for remap in (False, True):
+                s.set_launch(wpc, remap, lines)
+                out.zero_()
+                s.compute_rhs_dev(U, out, _lib.LAYOUT_LINES)
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), (lines, wpc, remap)
+    wpc, remap, lines, us = s.autotune(U, out, _lib.LAYOUT_LINES, reps=2)
+    assert 1 <= wpc <= 8 and lines in (128, 256, 512) and us > 0
+    assert s.get_launch() == (wpc, remap, lines)
     s.compute_rhs_dev(U, out, _lib.LAYOUT_LINES)
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
