@@ -817,7 +817,8 @@ def main():
             s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
         ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
                              args.steps, args.warmup, stream)
-        print(json.dumps({"kernel": "volume_kernel<4, float, float, EPI_RHS>",
+        print(json.dumps({"kernel": f"volume_kernel<4, float, float, EPI_RHS, "
+                                    f"{s32.get_launch()[2]}>",
                           "us": round(ms32 * 1e3, 2),
                           "gdof_s": round(dof * 5 / (ms32 * 1e-3) / 1e9, 3),
                           "launch": list(s32.get_launch())}), flush=True)
@@ -828,11 +829,13 @@ def main():
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
     R = torch.empty_like(U)
-    # launch shape from the library's autotuner (workgroups per CU, XCD order)
+    # launch shape from the library's autotuner (workgroups per CU, XCD order,
+    # grid lines per workgroup)
     if args.launch:
         s64.set_launch(*(int(v) for v in args.launch.split(",")))
         tune = s64.get_launch() + (float("nan"),)
-        log(f"launch shape fixed: {wpc} workgroups/CU, xcd_remap={bool(remap)}")
+        log(f"launch shape fixed: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, "
+            f"{tune[2]} lines per workgroup")
     else:
         tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
         log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]} lines per "
@@ -923,7 +926,8 @@ def main():
             "fp64_flop_frac": round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)
                               if flops else None,
             "algorithmic_bytes_per_launch": bytes_per,
-            "kernel": "volume_kernel<4, double, double, EPI_RHS>",
+            "kernel": f"volume_kernel<4, double, double, EPI_RHS, {tune[2]}>",
+            "launch": {"wg_per_cu": tune[0], "xcd_remap": tune[1], "wg_lines": tune[2]},
         },
         "cpu_baseline": cpu,
         "extras": extras,
