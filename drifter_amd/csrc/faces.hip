@@ -91,6 +91,146 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The conforming part of the face term, element by element: every interface
+// seen from each of its two elements, so that a thread adds to its own
+// element's nodes only.  eface[6 e + f] = (neighbour << 4) | (neighbour's face
+// << 1) | side for a conforming face (side 0: this element is the
+// interface's left, whose face gives the normal), -1 otherwise; a conforming
+// neighbour always meets the face with its opposite face f ^ 1.  One thread
+// per (element, field, xi-line): it reads and rewrites its line of U and dU
+// (contiguous across the workgroup in the LINES layout, each element's block
+// once) and reads the neighbours' nodes on the shared faces.
+// Per node the contributions of the x, y and z faces are summed (in that
+// order, the order of the interface lists) before M^-1 is applied
+// (operators_3d.cpp:277-281).  Each contribution is evaluated with the
+// interface's own left/right order, as the interface kernel adds it.
+// Addresses: a wave-uniform base per (field, xi-plane) and a 32-bit per-lane
+// byte offset (element and line), which the host checks fits.
+struct FaceNb {
+    uint32_t off; // byte offset of the neighbour's line, plane-relative
+    int side;
+    bool on;
+};
+
+// r = (side 0) 0 + wq F(u_own, u_nb, n_f) or (side 1) 0 - wq F(u_nb, u_own, n_(f^1))
+template <class T>
+__device__ __forceinline__ T face_contrib(const FaceArgs<T> &a, int f, int side, T own, T other,
+                                          T wq)
+{
+    const int fl = side ? (f ^ 1) : f;
+    T nrm[3] = {T(0), T(0), T(0)};
+    nrm[fl >> 1] = (fl & 1) ? T(1) : T(-1);
+    if (side == 0)
+        return T(0) + wq * scalar_flux_t<T>(a.kind, own, other, nrm, a.p);
+    return T(0) - wq * scalar_flux_t<T>(a.kind, other, own, nrm, a.p);
+}
+
+template <class T> __device__ __forceinline__ T ldb(const T *base, uint32_t off)
+{
+    return *(const T *)((const char *)base + off);
+}
+
+template <int n, class T>
+__global__ void __launch_bounds__(256)
+    face_elem_kernel(int64_t E, const int64_t *__restrict__ eface, FaceArgs<T> a)
+{
+    // one thread per (element, field, xi-line): every load of a thread is
+    // issued before its arithmetic, and a workgroup's lanes for one field
+    // and xi-plane read one contiguous run
+    constexpr int n2 = n * n, n3 = n2 * n, PER = 5 * n2, EPB = 256 / PER, NT = EPB * PER;
+    // face-rule weights by face DOF (0 past the N-point rule) and M^-1 by node
+    __shared__ T sWq[n2], sMinv[n3];
+    const int tid = threadIdx.x;
+    const int nq1 = a.nq1;
+    for (int q = tid; q < n2; q += blockDim.x)
+        sWq[q] = (q < nq1 * nq1) ? a.rw[q % nq1] * a.rw[q / nq1] : T(0);
+    for (int d = tid; d < n3; d += blockDim.x)
+        sMinv[d] = inv_mass(a, n, d);
+    __syncthreads();
+    if (tid >= NT)
+        return;
+    const int le = tid / PER, r = tid - le * PER, fld = r / n2, L = r - fld * n2;
+    const int j = L % n, k = L / n;
+    const int nq = nq1 * nq1;
+    const int64_t nbatch = (E + EPB - 1) / EPB;
+    const T *__restrict__ Uf = a.U + fld * a.fs;
+    T *__restrict__ dUf = a.dU + fld * a.fs;
+    auto line_off = [&](int64_t el, int line) {
+        return (uint32_t)((el * a.es + (int64_t)line * a.ls) * (int64_t)sizeof(T));
+    };
+    auto entry = [&](int64_t v, int line) {
+        FaceNb r;
+        r.on = v >= 0;
+        r.side = (int)(v & 1);
+        r.off = r.on ? line_off(v >> 4, line) : 0u;
+        return r;
+    };
+    // the faces this line lies on: both x-faces (its end nodes), a y-face
+    // when j is 0 or N, a z-face when k is 0 or N (every node of the line);
+    // the neighbour's line across a y-face has j = N - j, across a z-face
+    // k = N - k, across an x-face it is the same line
+    const int fy = (j == 0) ? 2 : (j == n - 1) ? 3 : -1;
+    const int fz = (k == 0) ? 4 : (k == n - 1) ? 5 : -1;
+    const int Ly = (n - 1 - j) + n * k, Lz = j + n * (n - 1 - k);
+    // XCD-aware order: workgroup w runs on XCD w mod 8, and XCD x takes the
+    // x-th eighth of the batches in order, so that neighbouring elements (a
+    // few Morton positions apart) are read behind one L2 (grid: a multiple
+    // of 8 workgroups)
+    const int64_t G = gridDim.x;
+    const int64_t lw = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8
+                                    : (int64_t)blockIdx.x;
+    for (int64_t b = lw; b < nbatch; b += G) {
+        const int64_t e = b * EPB + le;
+        if (e >= E)
+            continue;
+        const FaceNb x0 = entry(eface[6 * e], L), x1 = entry(eface[6 * e + 1], L);
+        const FaceNb y = entry(fy >= 0 ? eface[6 * e + fy] : -1, Ly);
+        const FaceNb z = entry(fz >= 0 ? eface[6 * e + fz] : -1, Lz);
+        const uint32_t own = line_off(e, L);
+        T u[n], du[n], ux0 = T(0), ux1 = T(0), uy[n], uz[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            u[i] = ldb(Uf + i * a.is, own);
+            du[i] = ldb(dUf + i * a.is, own);
+            uy[i] = y.on ? ldb(Uf + i * a.is, y.off) : T(0);
+            uz[i] = z.on ? ldb(Uf + i * a.is, z.off) : T(0);
+        }
+        if (x0.on)
+            ux0 = ldb(Uf + (n - 1) * a.is, x0.off);
+        if (x1.on)
+            ux1 = ldb(Uf, x1.off);
+        T acc[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            acc[i] = T(0);
+        // x-faces: node i = 0 (face 0) or N (face 1), face DOF (j, k) = L;
+        // the neighbour's node is at the other end of the same line
+        if (L < nq) {
+            const T wq = sWq[L];
+            if (x0.on)
+                acc[0] += face_contrib<T>(a, 0, x0.side, u[0], ux0, wq);
+            if (x1.on)
+                acc[n - 1] += face_contrib<T>(a, 1, x1.side, u[n - 1], ux1, wq);
+        }
+        // y-face: face DOF (i, k); z-face: face DOF (i, j); the same plane i
+        if (y.on) {
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                if (i + n * k < nq)
+                    acc[i] += face_contrib<T>(a, fy, y.side, u[i], uy[i], sWq[i + n * k]);
+        }
+        if (z.on) {
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                if (i + n * j < nq)
+                    acc[i] += face_contrib<T>(a, fz, z.side, u[i], uz[i], sWq[i + n * j]);
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            *(T *)((char *)(dUf + i * a.is) + own) = du[i] + sMinv[i + n * L] * acc[i];
+    }
+}
+
 // MortarSpace::compute_mortar_flux on the interfaces list[6 m .. 6 m + 5] =
 // (coarse element, coarse face, fine elements of sub-faces 0..NF-1), the fine
 // faces being the coarse face's opposite.  ops = [Minv | Pc | Lc | Pf[NF] |
@@ -205,6 +345,16 @@ hipError_t conf_n(bool lift, int64_t nif, const int64_t *list, const FaceArgs<T>
     return hipGetLastError();
 }
 
+template <int n, class T>
+hipError_t elem_n(int64_t E, const int64_t *eface, const FaceArgs<T> &a, hipStream_t s)
+{
+    constexpr int EPB = 256 / (5 * n * n);
+    const unsigned g8 = (grid_for(E, EPB, 1u << 20) + 7u) & ~7u; // a multiple of 8
+    const dim3 g(g8), b(256);
+    hipLaunchKernelGGL((face_elem_kernel<n, T>), g, b, 0, s, E, eface, a);
+    return hipGetLastError();
+}
+
 template <int n, int NF, class T>
 hipError_t mortar_nf(bool lift, int64_t nif, const int64_t *list, const T *ops,
                      const FaceArgs<T> &a, T *oc, T *of, hipStream_t s)
@@ -249,6 +399,21 @@ hipError_t launch_face_conforming(int n, bool lift, int64_t nif, const int64_t *
 }
 
 template <class T>
+hipError_t launch_face_elements(int n, int64_t E, const int64_t *eface, const FaceArgs<T> &a,
+                                hipStream_t s)
+{
+    if (E <= 0)
+        return hipSuccess;
+    switch (n) {
+    case 2: return elem_n<2, T>(E, eface, a, s);
+    case 3: return elem_n<3, T>(E, eface, a, s);
+    case 4: return elem_n<4, T>(E, eface, a, s);
+    case 5: return elem_n<5, T>(E, eface, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class T>
 hipError_t launch_face_mortar(int n, int nfine, bool lift, int64_t nif, const int64_t *list,
                               const T *ops, const FaceArgs<T> &a, T *out_c, T *out_f,
                               hipStream_t s)
@@ -270,6 +435,10 @@ template hipError_t launch_face_conforming<double>(int, bool, int64_t, const int
 template hipError_t launch_face_conforming<float>(int, bool, int64_t, const int64_t *,
                                                   const FaceArgs<float> &, float *, float *,
                                                   hipStream_t);
+template hipError_t launch_face_elements<double>(int, int64_t, const int64_t *,
+                                                 const FaceArgs<double> &, hipStream_t);
+template hipError_t launch_face_elements<float>(int, int64_t, const int64_t *,
+                                                const FaceArgs<float> &, hipStream_t);
 template hipError_t launch_face_mortar<double>(int, int, bool, int64_t, const int64_t *,
                                                const double *, const FaceArgs<double> &, double *,
                                                double *, hipStream_t);

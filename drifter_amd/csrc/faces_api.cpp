@@ -39,7 +39,11 @@ struct drf_face_plan {
     std::vector<int64_t> conf; // [nc][4]: left element, left face, right element, right face
     std::vector<int64_t> mort; // [nm][6]: coarse element, coarse face, fine elements by sub-face
     std::vector<int> mort_type;
-    int64_t *d_conf = nullptr, *d_mort = nullptr;
+    // [E][6]: (neighbour << 4) | (neighbour's face << 1) | side of every
+    // conforming face (side 0: the element is the interface's left), else -1
+    std::vector<int64_t> eface;
+    int64_t *d_conf = nullptr, *d_mort = nullptr, *d_eface = nullptr;
+    bool by_element = true; // conforming interfaces in one element-ordered launch
     void *d_ops[7] = {};       // per connection type: Minv | Pc | Lc | Pf[] | Lf[] (context precision)
     drf_ctx *attached = nullptr;
 };
@@ -116,6 +120,7 @@ void free_plan(drf_face_plan *p)
         p->attached->faces = nullptr;
     (void)hipFree(p->d_conf);
     (void)hipFree(p->d_mort);
+    (void)hipFree(p->d_eface);
     for (void *o : p->d_ops)
         (void)hipFree(o);
     delete p;
@@ -151,6 +156,19 @@ drf_status run_groups(const drf_face_plan *p, int group, const void *U, void *dU
     fill_args<T>(p, a, U, dU, layout);
     const bool lift = group < 0;
     size_t g0 = lift ? 0 : (size_t)group, g1 = lift ? p->groups.size() : (size_t)group + 1;
+    // The whole face term: every conforming interface in one element-ordered
+    // launch (each element's block read and rewritten once; its 32-bit lane
+    // offsets need the state's element span below 4 GiB, else the
+    // interface-ordered launches below take the conforming groups), then the
+    // mortar groups.
+    const int64_t span = ((p->E - 1) * a.es + (int64_t)(p->nh - 1) * a.ls) * (int64_t)sizeof(T);
+    const bool by_element =
+        lift && p->by_element && !p->conf.empty() && span < ((int64_t)1 << 32);
+    if (by_element) {
+        const hipError_t e = launch_face_elements<T>(p->n, p->E, p->d_eface, a, s);
+        if (e != hipSuccess)
+            return fail(DRF_ERR_HIP, std::string("face kernel launch: ") + hipGetErrorString(e));
+    }
     // consecutive groups of one (axis, kind) are one launch
     while (g0 < g1) {
         size_t ge = g0 + 1;
@@ -159,6 +177,10 @@ drf_status run_groups(const drf_face_plan *p, int group, const void *U, void *dU
                    p->groups[ge][1] == p->groups[g0][1])
                 ++ge;
         const int kind = (int)p->groups[g0][1];
+        if (by_element && kind == DRF_CONN_SAME_LEVEL) {
+            g0 = ge;
+            continue;
+        }
         const int64_t first = p->groups[g0][3];
         int64_t cnt = 0;
         for (size_t g = g0; g < ge; ++g)
@@ -316,11 +338,22 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
             p->mort_type.push_back(it.kind);
         }
     }
+    if (!p->conf.empty()) {
+        p->eface.assign((size_t)E * 6, -1);
+        for (size_t c = 0; c < p->conf.size(); c += 4) {
+            const int64_t el = p->conf[c], fl = p->conf[c + 1], er = p->conf[c + 2],
+                          fr = p->conf[c + 3];
+            p->eface[(size_t)(6 * el + fl)] = (er << 4) | (fr << 1);
+            p->eface[(size_t)(6 * er + fr)] = (el << 4) | (fl << 1) | 1;
+        }
+    }
     drf_status st = hipSetDevice(ctx->device) == hipSuccess
                         ? DRF_OK
                         : fail(DRF_ERR_HIP, "drf_face_plan_create: hipSetDevice failed");
     if (!st)
         st = upload(p->conf, &p->d_conf);
+    if (!st)
+        st = upload(p->eface, &p->d_eface);
     if (!st)
         st = upload(p->mort, &p->d_mort);
     // mortar operators per connection type present, in the context's precision
@@ -354,6 +387,13 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
 }
 
 void drf_face_plan_destroy(drf_face_plan *plan) { free_plan(plan); }
+
+drf_status drf_face_plan_set_conforming_order(drf_face_plan *p, int by_element)
+{
+    DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_plan_set_conforming_order: null plan");
+    p->by_element = by_element != 0;
+    return DRF_OK;
+}
 
 drf_status drf_face_plan_info(const drf_face_plan *p, int *n_groups, int64_t *n_conforming,
                               int64_t *n_mortar)

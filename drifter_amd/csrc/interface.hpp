@@ -93,6 +93,11 @@ template <class T> struct FaceArgs {
 template <class T>
 hipError_t launch_face_conforming(int n, bool lift, int64_t nif, const int64_t *list,
                                   const FaceArgs<T> &a, T *out_l, T *out_r, hipStream_t s);
+// The conforming face term element by element (eface: [E][6] packed
+// neighbour, neighbour face and side, -1 where the face is not conforming).
+template <class T>
+hipError_t launch_face_elements(int n, int64_t E, const int64_t *eface, const FaceArgs<T> &a,
+                                hipStream_t s);
 template <class T>
 hipError_t launch_face_mortar(int n, int nfine, bool lift, int64_t nif, const int64_t *list,
                               const T *ops, const FaceArgs<T> &a, T *out_c, T *out_f,

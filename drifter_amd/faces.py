@@ -96,6 +96,11 @@ class FacePlan:
         check(self._l.drf_face_plan_lists(self._h, cf.ctypes.data, mo.ctypes.data))
         return cf, mo
 
+    def set_conforming_order(self, by_element=True):
+        """Conforming interfaces in one element-ordered launch (the default)
+        or interface by interface, one launch per normal axis."""
+        check(self._l.drf_face_plan_set_conforming_order(self._h, int(bool(by_element))))
+
     def face_term(self, U, dU, layout=_lib.LAYOUT_LINES, stream=None):
         """dU += the face term of state U (device tensors of the context's precision)."""
         s = self.solver

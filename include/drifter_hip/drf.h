@@ -411,8 +411,13 @@ typedef struct drf_face_plan drf_face_plan;
  * for drf_flux_points) is applied to each of the five fields.  Interfaces are
  * grouped by normal axis, kind (DRF_CONN_SAME_LEVEL for conforming faces,
  * else the mortar's connection type) and level (the largest directional
- * level of the element owning the interface); each (axis, kind) runs as one
- * persistent kernel over its level groups.  Orders 1-4.
+ * level of the element owning the interface).  drf_face_term_dev runs every
+ * conforming interface in one element-ordered launch (each element adds the
+ * faces it owns a side of, its block read and rewritten once) and each
+ * mortar (axis, kind) as one persistent kernel over its level groups; with
+ * drf_face_plan_set_conforming_order(plan, 0), or when the state's element
+ * span exceeds 4 GiB, the conforming groups run interface by interface too,
+ * one launch per axis.  Orders 1-4.
  * DRF_ERR_UNSUPPORTED when a mortar's fine faces do not tile its coarse face
  * as its type says (the 1:3 fallback of get_connection_type);
  * DRF_ERR_ARG when a face would belong to two interfaces.  The plan keeps
@@ -422,6 +427,10 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
                                 const int64_t *elems, const int8_t *faces, int kind,
                                 const double *params, drf_face_plan **out);
 void drf_face_plan_destroy(drf_face_plan *plan);
+/* Conforming interfaces by element (1, the default) or by interface (0) in
+ * drf_face_term_dev and the attached right-hand side (no counterpart in the
+ * reference: the launch organisation; both give the face term to rounding). */
+drf_status drf_face_plan_set_conforming_order(drf_face_plan *plan, int by_element);
 drf_status drf_face_plan_info(const drf_face_plan *plan, int *n_groups, int64_t *n_conforming,
                               int64_t *n_mortar);
 /* per group, in launch order: desc[5g .. 5g+4] = axis, kind, level, first
