@@ -779,6 +779,9 @@ def main():
     ap.add_argument("--profile-mixed", action="store_true",
                     help="launch the mixed-precision RHS only, on the headline mesh (for "
                          "instruction and traffic counters of that kernel)")
+    ap.add_argument("--profile-faces", action="store_true",
+                    help="launch the face term only, FP64, on the headline mesh with every "
+                         "face conforming (for its counters)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time plain copies of known size instead (HBM counter calibration "
                          "and copy ceiling)")
@@ -805,6 +808,28 @@ def main():
     dof = E * nd
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream()
+
+    if args.profile_faces:
+        from drifter_amd import faces
+        from drifter_amd.mesh import OctreeAdapter
+
+        m = OctreeAdapter(0.0, 100000.0, 0.0, 100000.0, -1.0, 0.0)
+        m.build_uniform(128, 128, 16)
+        b, lv, _ = m.elements()
+        conn = m.build_face_connections(two_to_one=True)
+        s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
+        s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
+        plan = faces.FacePlan(s64, b, lv, conn, kind=faces.LF_ADV, params=(0.3, -0.2, 0.1, 0.6))
+        U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
+        R = torch.zeros_like(U)
+        msf = time_launches(lambda: plan.face_term(U, R, _lib.LAYOUT_LINES, stream),
+                            args.steps, args.warmup, stream)
+        ng, nc, nm = plan.info()
+        print(json.dumps({"kernel": "face_elem_kernel<5, double>", "us": round(msf * 1e3, 2),
+                          "conforming_interfaces": nc, "elements": E}), flush=True)
+        plan.close()
+        s64.close()
+        return
 
     if args.profile_mixed:
         s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)

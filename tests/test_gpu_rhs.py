@@ -111,13 +111,17 @@ def test_error_before_initialize():
     s.close()
 
 
+@pytest.mark.parametrize("order", [3, 4, 1])
 @pytest.mark.parametrize("prec", [_lib.PREC_FP64, _lib.PREC_MIXED])
-def test_launch_shape_does_not_change_results(prec):
+def test_launch_shape_does_not_change_results(prec, order):
     """every element is computed the same way whatever the grid: all launch
-    shapes, and the autotuner's choice, give identical bits"""
+    shapes (workgroup sizes, workgroups per CU, batch orders), and the
+    autotuner's choice, give identical bits; 13 x 9 x 4 elements at orders 1
+    and 4 leave partly filled last batches at every workgroup size"""
     import torch
 
-    case = synth.config1()
+    case = synth.config1() if order == 3 else synth.make_case(13, 9, 4, order, 10000.0,
+                                                              "seamount")
     E, n = case.E, case.n
     s = PrimitiveEquationsSolver(case.order, prec)
     s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
