@@ -76,7 +76,7 @@ SIGNATURES = {
     "drf_mesh_face_connections_2to1": (_I, [_P, _P, _P, _P, _P, _P]),
     "drf_face_plan_create": (_I, [_P, _L, _P, _P, _P, _P, _P, _P, _I, _PD, C.POINTER(_P)]),
     "drf_face_plan_destroy": (None, [_P]),
-    "drf_face_plan_set_conforming_order": (_I, [_P, _I]),
+    "drf_face_plan_set_conforming_mode": (_I, [_P, _I]),
     "drf_face_plan_info": (_I, [_P, C.POINTER(_I), C.POINTER(_L), C.POINTER(_L)]),
     "drf_face_plan_groups": (_I, [_P, _P]),
     "drf_face_plan_lists": (_I, [_P, _P, _P]),

@@ -130,10 +130,12 @@ bool lane_offsets_fit(const drf_ctx *c, int layout)
 }
 
 // One volume-kernel launch at the context's precision.
+// faces: the context precision's VolFaces (the fused conforming face term
+// of the plain RHS) or null.
 drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode, void *R,
                   const void *U0, void *Y, void *A, const void *X, double c0, double c1,
                   double c2, double c3, hipStream_t s, const double *dtp = nullptr,
-                  double dtq = 1.0)
+                  double dtq = 1.0, const void *faces = nullptr)
 {
     DRF_REQUIRE(lane_offsets_fit(c, layout), DRF_ERR_UNSUPPORTED,
                 "mesh too large for 32-bit per-field offsets in this layout (use LINES or SOA)");
@@ -155,6 +157,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.dtq = dtq;
         VolGeom<double> geo{c->E, coef_view<double>(c->d_h, c), coef_view<double>(c->d_f, c),
                             c->f0, c->wg_per_cu, c->xcd_remap, c->wg_lines};
+        if (faces)
+            geo.faces = *(const VolFaces<double> *)faces;
         err = launch_volume_fp64(c->order, geo, c->ops, c->phys, cview<double>(U, layout, c), epi,
                                  s);
     } else {
@@ -174,6 +178,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
         epi.dtq = dtq;
         VolGeom<float> geo{c->E, coef_view<float>(c->d_h32, c), coef_view<float>(c->d_f32, c),
                            c->f0, c->wg_per_cu, c->xcd_remap, c->wg_lines};
+        if (faces)
+            geo.faces = *(const VolFaces<float> *)faces;
         err = launch_volume_fp32(c->order, geo, c->ops, c->phys, cview<float>(U, layout, c), epi,
                                  s);
     }
@@ -184,11 +190,20 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
 
 drf_status rhs_into(drf_ctx *c, const void *U, void *R, int layout, hipStream_t s)
 {
-    drf_status st =
-        volume(c, U, layout, EPI_RHS, 0, R, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0, s);
+    // An attached face plan's conforming interfaces go into the volume
+    // kernel's last phase where the plan allows it; its mortars follow.
+    VolFaces<double> f64{};
+    VolFaces<float> f32{};
+    const bool fused = c->faces && (c->precision == DRF_PREC_FP64
+                                        ? volume_faces<double>(c->faces, layout, &f64)
+                                        : volume_faces<float>(c->faces, layout, &f32));
+    const void *vf = !fused ? nullptr : c->precision == DRF_PREC_FP64 ? (const void *)&f64
+                                                                       : (const void *)&f32;
+    drf_status st = volume(c, U, layout, EPI_RHS, 0, R, nullptr, nullptr, nullptr, nullptr, 0, 0,
+                           0, 0, s, nullptr, 1.0, vf);
     if (st || !c->faces)
         return st;
-    return face_term(c->faces, U, R, layout, s);
+    return fused ? face_term_mortars(c->faces, U, R, layout, s) : face_term(c->faces, U, R, layout, s);
 }
 
 // Un-fused stepping: the reference's sequence of RHS evaluations and vector

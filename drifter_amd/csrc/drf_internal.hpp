@@ -104,6 +104,17 @@ template <class T> struct Epi {
     double dtq;
 };
 
+// The conforming face term fused into the volume kernel's last phase (a
+// face plan's element table; eface == nullptr: none).  Per element face:
+// (neighbour << 4) | (neighbour's face << 1) | side, -1 where not conforming.
+template <class T> struct VolFaces {
+    const int64_t *eface;
+    const T *wq; // face-rule weight by face DOF, (N+1)^2 values, 0 past the N-point rule
+    T p[4];      // scalar flux parameters (vx, vy, vz, lambda)
+    int kind;    // DRF_FLUX_UPWIND or DRF_FLUX_LF_ADV
+    int nq;      // face DOFs that carry the N-point rule
+};
+
 template <class T> struct VolGeom {
     int64_t E;
     View<const T> h; // bathymetry at every node (fs unused)
@@ -112,6 +123,7 @@ template <class T> struct VolGeom {
     int wg_per_cu;   // launch shape: resident workgroups per CU (0: occupancy limit)
     int xcd_remap;   // renumber workgroups so neighbouring batches share an XCD
     int wg_lines;    // grid lines per workgroup: 128, 256 or 512 (orders above 4: 128)
+    VolFaces<T> faces; // conforming face term added in the plain RHS (orders 1-4)
 };
 
 // Launchers, one translation unit per workgroup size (volume_fp64*.hip,
@@ -173,6 +185,11 @@ hipError_t launch_adaptive_dt(const double *cmax, double dx_min, double cfl, dou
 // Face term of an attached face plan (faces_api.cpp): dU += the plan's
 // conforming and mortar contributions.
 drf_status face_term(const drf_face_plan *p, const void *U, void *dU, int layout, hipStream_t s);
+// The conforming part as the volume kernel's fused phase (false when the
+// plan keeps it out of the volume kernel), and the mortar groups alone.
+template <class T> bool volume_faces(const drf_face_plan *p, int layout, VolFaces<T> *out);
+drf_status face_term_mortars(const drf_face_plan *p, const void *U, void *dU, int layout,
+                             hipStream_t s);
 
 // Device address of the file-scope scratch array of noalloc.hip, looked up
 // once per device at context creation (the device must be current).

@@ -43,7 +43,9 @@ struct drf_face_plan {
     // conforming face (side 0: the element is the interface's left), else -1
     std::vector<int64_t> eface;
     int64_t *d_conf = nullptr, *d_mort = nullptr, *d_eface = nullptr;
-    bool by_element = true; // conforming interfaces in one element-ordered launch
+    void *d_wq = nullptr;   // face-rule weight by face DOF (plan precision), for the fused phase
+    // conforming interfaces: DRF_FACES_BY_INTERFACE, _BY_ELEMENT or _IN_VOLUME
+    int conf_mode = DRF_FACES_IN_VOLUME;
     void *d_ops[7] = {};       // per connection type: Minv | Pc | Lc | Pf[] | Lf[] (context precision)
     drf_ctx *attached = nullptr;
 };
@@ -121,6 +123,7 @@ void free_plan(drf_face_plan *p)
     (void)hipFree(p->d_conf);
     (void)hipFree(p->d_mort);
     (void)hipFree(p->d_eface);
+    (void)hipFree(p->d_wq);
     for (void *o : p->d_ops)
         (void)hipFree(o);
     delete p;
@@ -147,24 +150,32 @@ template <class T> void fill_args(const drf_face_plan *p, FaceArgs<T> &a, const 
         a.p[i] = (T)p->params[i];
 }
 
-// One group (or, with group < 0, every group with LIFT) of a plan.
+// The element-ordered conforming launch (standalone or fused) addresses the
+// state through 32-bit per-lane byte offsets: the element span must fit.
+template <class T> bool element_offsets_fit(const drf_face_plan *p, const FaceArgs<T> &a)
+{
+    const int64_t span = ((p->E - 1) * a.es + (int64_t)(p->nh - 1) * a.ls) * (int64_t)sizeof(T);
+    return span < ((int64_t)1 << 32);
+}
+
+// One group (or, with group < 0, every group with LIFT) of a plan; with
+// skip_conforming the conforming groups are left out (the volume kernel has
+// added them).
 template <class T>
 drf_status run_groups(const drf_face_plan *p, int group, const void *U, void *dU, int layout,
-                      void *out_a, void *out_b, hipStream_t s)
+                      void *out_a, void *out_b, hipStream_t s, bool skip_conforming = false)
 {
     FaceArgs<T> a;
     fill_args<T>(p, a, U, dU, layout);
     const bool lift = group < 0;
     size_t g0 = lift ? 0 : (size_t)group, g1 = lift ? p->groups.size() : (size_t)group + 1;
     // The whole face term: every conforming interface in one element-ordered
-    // launch (each element's block read and rewritten once; its 32-bit lane
-    // offsets need the state's element span below 4 GiB, else the
-    // interface-ordered launches below take the conforming groups), then the
-    // mortar groups.
-    const int64_t span = ((p->E - 1) * a.es + (int64_t)(p->nh - 1) * a.ls) * (int64_t)sizeof(T);
-    const bool by_element =
-        lift && p->by_element && !p->conf.empty() && span < ((int64_t)1 << 32);
-    if (by_element) {
+    // launch (each element's block read and rewritten once; when the state's
+    // element span exceeds 4 GiB the interface-ordered launches below take
+    // the conforming groups), then the mortar groups.
+    const bool by_element = lift && p->conf_mode != DRF_FACES_BY_INTERFACE &&
+                            !p->conf.empty() && element_offsets_fit<T>(p, a);
+    if (by_element && !skip_conforming) {
         const hipError_t e = launch_face_elements<T>(p->n, p->E, p->d_eface, a, s);
         if (e != hipSuccess)
             return fail(DRF_ERR_HIP, std::string("face kernel launch: ") + hipGetErrorString(e));
@@ -177,7 +188,7 @@ drf_status run_groups(const drf_face_plan *p, int group, const void *U, void *dU
                    p->groups[ge][1] == p->groups[g0][1])
                 ++ge;
         const int kind = (int)p->groups[g0][1];
-        if (by_element && kind == DRF_CONN_SAME_LEVEL) {
+        if ((by_element || skip_conforming) && kind == DRF_CONN_SAME_LEVEL) {
             g0 = ge;
             continue;
         }
@@ -213,6 +224,38 @@ drf_status face_term(const drf_face_plan *p, const void *U, void *dU, int layout
     if (p->precision == DRF_PREC_FP64)
         return run_groups<double>(p, -1, U, dU, layout, nullptr, nullptr, s);
     return run_groups<float>(p, -1, U, dU, layout, nullptr, nullptr, s);
+}
+
+// The fused form: the conforming part for the volume kernel (false when the
+// plan keeps it out: another mode, no conforming interface, or offsets that
+// do not fit), and the mortar groups on their own.
+template <class T> bool volume_faces(const drf_face_plan *p, int layout, VolFaces<T> *out)
+{
+    if (p->conf_mode != DRF_FACES_IN_VOLUME || p->conf.empty() || !p->d_wq)
+        return false;
+    FaceArgs<T> a;
+    fill_args<T>(p, a, nullptr, nullptr, layout);
+    if (!element_offsets_fit<T>(p, a))
+        return false;
+    out->eface = p->d_eface;
+    out->wq = (const T *)p->d_wq;
+    for (int i = 0; i < 4; ++i)
+        out->p[i] = a.p[i];
+    out->kind = p->kind;
+    out->nq = a.nq1 * a.nq1;
+    return true;
+}
+template bool volume_faces<double>(const drf_face_plan *, int, VolFaces<double> *);
+template bool volume_faces<float>(const drf_face_plan *, int, VolFaces<float> *);
+
+drf_status face_term_mortars(const drf_face_plan *p, const void *U, void *dU, int layout,
+                             hipStream_t s)
+{
+    if (p->groups.empty())
+        return DRF_OK;
+    if (p->precision == DRF_PREC_FP64)
+        return run_groups<double>(p, -1, U, dU, layout, nullptr, nullptr, s, true);
+    return run_groups<float>(p, -1, U, dU, layout, nullptr, nullptr, s, true);
 }
 
 } // namespace drf
@@ -354,6 +397,24 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
         st = upload(p->conf, &p->d_conf);
     if (!st)
         st = upload(p->eface, &p->d_eface);
+    // face-rule weight by face DOF, as the conforming kernels form it, in the
+    // plan's precision (the fused volume phase reads it)
+    if (!st && !p->conf.empty()) {
+        const int nq1 = p->order < 2 ? 2 : p->order, nh = p->nh;
+        double rx[kMaxN], rw[kMaxN];
+        lgl_nodes(nq1, rx, rw);
+        if (p->precision == DRF_PREC_FP64) {
+            std::vector<double> wq(nh, 0.0);
+            for (int q = 0; q < nq1 * nq1 && q < nh; ++q)
+                wq[q] = rw[q % nq1] * rw[q / nq1];
+            st = upload(wq, (double **)&p->d_wq);
+        } else {
+            std::vector<float> wq(nh, 0.0f);
+            for (int q = 0; q < nq1 * nq1 && q < nh; ++q)
+                wq[q] = (float)rw[q % nq1] * (float)rw[q / nq1];
+            st = upload(wq, (float **)&p->d_wq);
+        }
+    }
     if (!st)
         st = upload(p->mort, &p->d_mort);
     // mortar operators per connection type present, in the context's precision
@@ -388,10 +449,13 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
 
 void drf_face_plan_destroy(drf_face_plan *plan) { free_plan(plan); }
 
-drf_status drf_face_plan_set_conforming_order(drf_face_plan *p, int by_element)
+drf_status drf_face_plan_set_conforming_mode(drf_face_plan *p, int mode)
 {
-    DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_plan_set_conforming_order: null plan");
-    p->by_element = by_element != 0;
+    DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_plan_set_conforming_mode: null plan");
+    DRF_REQUIRE(mode == DRF_FACES_BY_INTERFACE || mode == DRF_FACES_BY_ELEMENT ||
+                    mode == DRF_FACES_IN_VOLUME,
+                DRF_ERR_ARG, "drf_face_plan_set_conforming_mode: unknown mode");
+    p->conf_mode = mode;
     return DRF_OK;
 }
 

@@ -33,6 +33,7 @@
 #include <type_traits>
 
 #include "drf_internal.hpp"
+#include "interface.hpp"
 
 namespace drf {
 namespace vol {
@@ -244,6 +245,96 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
     }
 }
 
+// The conforming face term of one element's xi-line, added to the line of
+// the tendencies the thread has just written (the fused form of
+// face_elem_kernel, faces.hip, with the same arithmetic and order: per node
+// the x, y and z contributions summed, then M^-1; each interface with its own
+// left/right flux order).  The thread rereads its own writes, so no barrier
+// is needed.
+template <class T>
+__device__ __forceinline__ T face_contrib_v(int kind, const T *prm, int f, int side, T own,
+                                            T other, T wq)
+{
+#pragma clang fp contract(off)
+    const int fl = side ? (f ^ 1) : f;
+    T nrm[3] = {T(0), T(0), T(0)};
+    nrm[fl >> 1] = (fl & 1) ? T(1) : T(-1);
+    if (side == 0)
+        return T(0) + wq * scalar_flux_t<T>(kind, own, other, nrm, prm);
+    return T(0) - wq * scalar_flux_t<T>(kind, other, own, nrm, prm);
+}
+
+template <int n, class TS>
+__device__ __forceinline__ void face_phase(const VolFaces<TS> &fc, const View<const TS> &U,
+                                           const View<TS> &R, int64_t e, int L, const TS *sWq,
+                                           const TS *sMinv)
+{
+#pragma clang fp contract(off)
+    struct Nb {
+        uint32_t off;
+        int side;
+        bool on;
+    };
+    const int j = L % n, k = L / n, nq = fc.nq;
+    // the faces this line lies on, and the neighbour's line across each (as
+    // face_elem_kernel: x-faces at its ends, y / z when j / k is 0 or N)
+    const int fy = (j == 0) ? 2 : (j == n - 1) ? 3 : -1;
+    const int fz = (k == 0) ? 4 : (k == n - 1) ? 5 : -1;
+    const int Ly = (n - 1 - j) + n * k, Lz = j + n * (n - 1 - k);
+    auto entry = [&](int f, int line) {
+        const int64_t v = f >= 0 ? fc.eface[6 * e + f] : -1;
+        Nb r;
+        r.on = v >= 0;
+        r.side = (int)(v & 1);
+        r.off = r.on ? lane_off(U, v >> 4, line) : 0u;
+        return r;
+    };
+    const Nb x0 = entry(0, L), x1 = entry(1, L), y = entry(fy, Ly), z = entry(fz, Lz);
+    const uint32_t ou = lane_off(U, e, L), orr = lane_off(R, e, L);
+#pragma unroll 1
+    for (int fld = 0; fld < 5; ++fld) {
+        const TS *Uf = U.p + fld * U.fs;
+        TS *Rf = R.p + fld * R.fs;
+        TS u[n], du[n], uy[n], uz[n], ux0 = TS(0), ux1 = TS(0), acc[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            u[i] = ldo<TS, TS>(Uf + i * U.is, ou);
+            du[i] = ldo<TS, TS>(Rf + i * R.is, orr);
+            uy[i] = y.on ? ldo<TS, TS>(Uf + i * U.is, y.off) : TS(0);
+            uz[i] = z.on ? ldo<TS, TS>(Uf + i * U.is, z.off) : TS(0);
+            acc[i] = TS(0);
+        }
+        if (x0.on)
+            ux0 = ldo<TS, TS>(Uf + (n - 1) * U.is, x0.off);
+        if (x1.on)
+            ux1 = ldo<TS, TS>(Uf, x1.off);
+        if (L < nq) {
+            const TS wq = sWq[L];
+            if (x0.on)
+                acc[0] += face_contrib_v<TS>(fc.kind, fc.p, 0, x0.side, u[0], ux0, wq);
+            if (x1.on)
+                acc[n - 1] += face_contrib_v<TS>(fc.kind, fc.p, 1, x1.side, u[n - 1], ux1, wq);
+        }
+        if (y.on) {
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                if (i + n * k < nq)
+                    acc[i] += face_contrib_v<TS>(fc.kind, fc.p, fy, y.side, u[i], uy[i],
+                                                 sWq[i + n * k]);
+        }
+        if (z.on) {
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+                if (i + n * j < nq)
+                    acc[i] += face_contrib_v<TS>(fc.kind, fc.p, fz, z.side, u[i], uz[i],
+                                                 sWq[i + n * j]);
+        }
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+            sto<TS>(Rf + i * R.is, orr, (TS)(du[i] + sMinv[i + n * L] * acc[i]));
+    }
+}
+
 // The kernel.  Seven LDS slots per element node, six of them reused in two
 // rounds (momentum, then tracers), so a workgroup of 5 N=4 elements needs
 // 35 KB of staging plus 4 KB of column integrals, and the CU's occupancy is
@@ -259,7 +350,8 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
 //   P23b eta-owner eta-derivatives of the tracer fluxes    -> A, B
 //        zeta-owner vertical diffusion of T, S (same phase) -> C, D
 //   P4b xi-owner   dHT, dHS and d eta complete (written out)
-template <int N, class TS, class TC, int EPI, int LW>
+//   P5  (FACES) xi-owner: the conforming face term added to its line
+template <int N, class TS, class TC, int EPI, int LW, bool FACES>
 __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_waves_per_eu(Cfg<N, LW>::template wpe<TS, EPI>())))
     volume_kernel(const VolGeom<TS> geo, const KOps<N + 1, TC> ops, const Phys ph,
                   const View<const TS> U, const Epi<TS> epi, const int64_t nbatch)
@@ -277,6 +369,8 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
     // kernel arguments such an index compiles to a global load, and its wait
     // would drain the next batch's prefetch and this batch's stores)
     __shared__ double sDc[n * n];
+    // FACES: face-rule weight by face DOF and M^-1 by node
+    __shared__ TS sWq[FACES ? n2 : 1], sMinv[FACES ? n3 : 1];
 
     // The plain RHS runs every lane through every phase, so that no phase
     // sits behind an exec-mask branch: lanes past the tile repeat the last
@@ -319,6 +413,17 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
     int64_t b = lw;
     if (tid < n * n)
         sDc[tid] = Dc[tid]; // first read in P4b, after P1's barrier
+    if constexpr (FACES) {
+        for (int q = tid; q < n2; q += blockDim.x)
+            sWq[q] = geo.faces.wq[q];
+        for (int d = tid; d < n3; d += blockDim.x) {
+            // 1 / (w_i w_j w_k), w = 2 sw exactly, rounded to the state type
+            const TS wi = (TS)(2.0 * ops.sw[d % n]), wj = (TS)(2.0 * ops.sw[(d / n) % n]),
+                     wk = (TS)(2.0 * ops.sw[d / n2]);
+            const TS w = wi * wj * wk;
+            sMinv[d] = TS(1) / w;
+        }
+    }
     Lines<n, TC> nxt;
     if (PREFETCH) {
         const int64_t e0 = elem(b);
@@ -594,6 +699,13 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
             }
             write_fields<n, 2, 3, TS, TC, EPI>(epi, e, L, kv);
         }
+        // ---------------- P5 (FACES): the conforming face term ----------------
+        // Not by the lanes that repeat element E - 1 from another wavefront:
+        // their rereads could meet the owner's final values.
+        if constexpr (FACES) {
+            if (b * EPB + le < geo.E)
+                face_phase<n, TS>(geo.faces, U, epi.R, e, L, sWq, sMinv);
+        }
         // No barrier here: this phase reads only its own line's slots and the
         // column integrals of this batch's parity; the next batch's P1 writes
         // its own line's slots and the other parity.
@@ -631,7 +743,7 @@ inline int num_cus()
     return cus;
 }
 
-template <int N, class TS, class TC, int EPI, int LW>
+template <int N, class TS, class TC, int EPI, int LW, bool FACES>
 hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
                      const Epi<TS> &epi, hipStream_t s)
 {
@@ -639,7 +751,7 @@ hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, Vie
     if (geo.E <= 0)
         return hipSuccess;
     const int64_t nbatch = (geo.E + C::EPB - 1) / C::EPB;
-    auto kern = volume_kernel<N, TS, TC, EPI, LW>;
+    auto kern = volume_kernel<N, TS, TC, EPI, LW, FACES>;
     // resident workgroups per CU (queried once per instantiation: the launch
     // path stays free of runtime queries)
     static const int per_cu = [&] {
@@ -664,13 +776,16 @@ template <int N, class TS, class TC, int LW>
 hipError_t launch_n(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, View<const TS> U,
                     const Epi<TS> &epi, hipStream_t s)
 {
+    if constexpr (N <= 4) // face plans are built for orders 1-4
+        if (epi.mode == EPI_RHS && geo.faces.eface)
+            return launch_ne<N, TS, TC, EPI_RHS, LW, true>(geo, ops, ph, U, epi, s);
     switch (epi.mode) {
-    case EPI_RHS: return launch_ne<N, TS, TC, EPI_RHS, LW>(geo, ops, ph, U, epi, s);
-    case EPI_STAGE: return launch_ne<N, TS, TC, EPI_STAGE, LW>(geo, ops, ph, U, epi, s);
-    case EPI_STAGE_ACC: return launch_ne<N, TS, TC, EPI_STAGE_ACC, LW>(geo, ops, ph, U, epi, s);
-    case EPI_RK4FIN: return launch_ne<N, TS, TC, EPI_RK4FIN, LW>(geo, ops, ph, U, epi, s);
-    case EPI_COMB: return launch_ne<N, TS, TC, EPI_COMB, LW>(geo, ops, ph, U, epi, s);
-    case EPI_LSRK: return launch_ne<N, TS, TC, EPI_LSRK, LW>(geo, ops, ph, U, epi, s);
+    case EPI_RHS: return launch_ne<N, TS, TC, EPI_RHS, LW, false>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE: return launch_ne<N, TS, TC, EPI_STAGE, LW, false>(geo, ops, ph, U, epi, s);
+    case EPI_STAGE_ACC: return launch_ne<N, TS, TC, EPI_STAGE_ACC, LW, false>(geo, ops, ph, U, epi, s);
+    case EPI_RK4FIN: return launch_ne<N, TS, TC, EPI_RK4FIN, LW, false>(geo, ops, ph, U, epi, s);
+    case EPI_COMB: return launch_ne<N, TS, TC, EPI_COMB, LW, false>(geo, ops, ph, U, epi, s);
+    case EPI_LSRK: return launch_ne<N, TS, TC, EPI_LSRK, LW, false>(geo, ops, ph, U, epi, s);
     default: return hipErrorInvalidValue;
     }
 }

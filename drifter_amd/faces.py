@@ -25,6 +25,7 @@ from ._lib import check
 from .solver import _dev_array, _sptr
 
 UPWIND, LF_ADV = 5, 6
+BY_INTERFACE, BY_ELEMENT, IN_VOLUME = 0, 1, 2
 KIND_CONFORMING = 0
 
 
@@ -96,10 +97,13 @@ class FacePlan:
         check(self._l.drf_face_plan_lists(self._h, cf.ctypes.data, mo.ctypes.data))
         return cf, mo
 
-    def set_conforming_order(self, by_element=True):
-        """Conforming interfaces in one element-ordered launch (the default)
-        or interface by interface, one launch per normal axis."""
-        check(self._l.drf_face_plan_set_conforming_order(self._h, int(bool(by_element))))
+    def set_conforming_mode(self, mode=None):
+        """How the conforming interfaces run: IN_VOLUME (the default: inside
+        the volume kernel of an attached right-hand side, element by element
+        in face_term), BY_ELEMENT (one element-ordered launch) or
+        BY_INTERFACE (interface by interface, one launch per normal axis)."""
+        check(self._l.drf_face_plan_set_conforming_mode(
+            self._h, IN_VOLUME if mode is None else int(mode)))
 
     def face_term(self, U, dU, layout=_lib.LAYOUT_LINES, stream=None):
         """dU += the face term of state U (device tensors of the context's precision)."""

@@ -414,10 +414,11 @@ typedef struct drf_face_plan drf_face_plan;
  * level of the element owning the interface).  drf_face_term_dev runs every
  * conforming interface in one element-ordered launch (each element adds the
  * faces it owns a side of, its block read and rewritten once) and each
- * mortar (axis, kind) as one persistent kernel over its level groups; with
- * drf_face_plan_set_conforming_order(plan, 0), or when the state's element
- * span exceeds 4 GiB, the conforming groups run interface by interface too,
- * one launch per axis.  Orders 1-4.
+ * mortar (axis, kind) as one persistent kernel over its level groups; an
+ * attached plan's conforming interfaces are added inside the volume kernel
+ * (drf_face_plan_set_conforming_mode).  When the state's element span exceeds
+ * 4 GiB the conforming groups run interface by interface, one launch per
+ * axis.  Orders 1-4.
  * DRF_ERR_UNSUPPORTED when a mortar's fine faces do not tile its coarse face
  * as its type says (the 1:3 fallback of get_connection_type);
  * DRF_ERR_ARG when a face would belong to two interfaces.  The plan keeps
@@ -427,10 +428,15 @@ drf_status drf_face_plan_create(const drf_ctx *ctx, int64_t n_elem, const double
                                 const int64_t *elems, const int8_t *faces, int kind,
                                 const double *params, drf_face_plan **out);
 void drf_face_plan_destroy(drf_face_plan *plan);
-/* Conforming interfaces by element (1, the default) or by interface (0) in
- * drf_face_term_dev and the attached right-hand side (no counterpart in the
- * reference: the launch organisation; both give the face term to rounding). */
-drf_status drf_face_plan_set_conforming_order(drf_face_plan *plan, int by_element);
+/* How the conforming interfaces run (no counterpart in the reference: the
+ * launch organisation; every mode gives the face term to rounding):
+ * DRF_FACES_IN_VOLUME (the default) adds them in the volume kernel's last
+ * phase when the plan is attached to a right-hand side, and element by
+ * element in drf_face_term_dev; DRF_FACES_BY_ELEMENT uses the element-ordered
+ * launch after the volume kernel; DRF_FACES_BY_INTERFACE one launch per normal
+ * axis, interface by interface.  Mortars always run as their own launches. */
+enum { DRF_FACES_BY_INTERFACE = 0, DRF_FACES_BY_ELEMENT = 1, DRF_FACES_IN_VOLUME = 2 };
+drf_status drf_face_plan_set_conforming_mode(drf_face_plan *plan, int mode);
 drf_status drf_face_plan_info(const drf_face_plan *plan, int *n_groups, int64_t *n_conforming,
                               int64_t *n_mortar);
 /* per group, in launch order: desc[5g .. 5g+4] = axis, kind, level, first

@@ -166,15 +166,15 @@ def faces_mortar(order, kind):
     return itf.MortarSpace(order, kind)
 
 
-@pytest.mark.parametrize("by_element", [True, False])
+@pytest.mark.parametrize("mode", [faces.IN_VOLUME, faces.BY_INTERFACE])
 @pytest.mark.parametrize("two", [True, False])
-def test_face_term_fp64_against_oracle(orc, coast, two, by_element):
+def test_face_term_fp64_against_oracle(orc, coast, two, mode):
     import torch
 
     case, mesh = coast
     s = _solver(case)
     plan = faces.FacePlan.from_mesh(s, mesh, two_to_one=two, kind=faces.LF_ADV, params=PARAMS)
-    plan.set_conforming_order(by_element)
+    plan.set_conforming_mode(mode)
     E, n = case.E, case.n
     U = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda()
     dU = torch.zeros_like(U)
@@ -184,21 +184,21 @@ def test_face_term_fp64_against_oracle(orc, coast, two, by_element):
     want = orc.face_term(case.order, E, cf, mt, _mort_types(plan.groups(), mt.shape[0]),
                          faces.LF_ADV, PARAMS, case.U)
     errs = rel_l2_fields(got, want, case.nd)
-    print(f"[faces] FP64 face term (two_to_one={two}, by_element={by_element}) rel L2 "
+    print(f"[faces] FP64 face term (two_to_one={two}, mode={mode}) rel L2 "
           f"{['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) <= TOL64, errs
     plan.close()
     s.close()
 
 
-@pytest.mark.parametrize("by_element", [True, False])
-def test_face_term_fp32_against_oracle(orc, coast, by_element):
+@pytest.mark.parametrize("mode", [faces.IN_VOLUME, faces.BY_INTERFACE])
+def test_face_term_fp32_against_oracle(orc, coast, mode):
     import torch
 
     case, mesh = coast
     s = _solver(case, _lib.PREC_MIXED)
     plan = faces.FacePlan.from_mesh(s, mesh, two_to_one=True, kind=faces.LF_ADV, params=PARAMS)
-    plan.set_conforming_order(by_element)
+    plan.set_conforming_mode(mode)
     E, n = case.E, case.n
     U = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda().float()
     dU = torch.zeros_like(U)
@@ -238,13 +238,13 @@ def test_face_term_conforming_orders_agree(coast, layout, kind):
     U = torch.from_numpy(host).cuda()
     base = torch.from_numpy(np.sin(np.arange(host.size) * 0.37)).cuda()
     got = {}
-    for by_element in (True, False):
-        plan.set_conforming_order(by_element)
+    for mode in (faces.BY_ELEMENT, faces.BY_INTERFACE):
+        plan.set_conforming_mode(mode)
         dU = base.clone()
         plan.face_term(U, dU, layout)
         torch.cuda.synchronize()
-        got[by_element] = dU.cpu().numpy()
-    a, b = got[True], got[False]
+        got[mode] = dU.cpu().numpy()
+    a, b = got[faces.BY_ELEMENT], got[faces.BY_INTERFACE]
     if layout == _lib.LAYOUT_LINES:
         a, b = synth.from_lines(a, E, n), synth.from_lines(b, E, n)
     elif layout == _lib.LAYOUT_SOA:
@@ -267,6 +267,40 @@ def _rhs_with_faces(orc, case, U, cf, mt, mtypes):
     f = np.full(case.h.size, 1e-4)
     r = orc.rhs(case.order, U, case.h, f)
     return orc.face_term(case.order, case.E, cf, mt, mtypes, faces.LF_ADV, PARAMS, U, r)
+
+
+@pytest.mark.parametrize("prec", [_lib.PREC_FP64, _lib.PREC_MIXED])
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_LINES, _lib.LAYOUT_SOA])
+def test_rhs_faces_in_volume_kernel_bit_identical(coast, prec, layout):
+    """The conforming face term added in the volume kernel's last phase gives
+    the same bits as the volume kernel followed by the element-ordered face
+    launch (the same arithmetic in the same order), at every workgroup size,
+    FP64 and mixed; the mortars run as their own launches in both."""
+    import torch
+
+    case, mesh = coast
+    s = _solver(case, prec)
+    plan = faces.FacePlan.from_mesh(s, mesh, two_to_one=True, kind=faces.LF_ADV, params=PARAMS)
+    E, n, nd = case.E, case.n, case.nd
+    host = synth.to_lines(case.U, E, n) if layout == _lib.LAYOUT_LINES else \
+        synth.to_soa(case.U, E, nd)
+    dt = torch.float64 if prec == _lib.PREC_FP64 else torch.float32
+    U = torch.from_numpy(host).cuda().to(dt)
+    plan.attach()
+    plan.set_conforming_mode(faces.BY_ELEMENT)
+    ref = torch.empty_like(U)
+    s.compute_rhs_dev(U, ref, layout)
+    plan.set_conforming_mode(faces.IN_VOLUME)
+    out = torch.empty_like(U)
+    for lines in (128, 256, 512):
+        s.set_launch(0, True, lines)
+        out.fill_(float("nan"))
+        s.compute_rhs_dev(U, out, layout)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), lines
+    plan.detach()
+    plan.close()
+    s.close()
 
 
 def test_rhs_and_unfused_rk4_with_faces(orc, coast):
