@@ -406,9 +406,10 @@ def measure_coastline(args, stream):
 def measure_faces_coastline(args, stream):
     """The batched adaptive DG right-hand side on the coastline octree
     (39,280 elements, N=3): the face term of the 2:1 plan (103,116 conforming
-    interfaces and 3,012 Fine2x1 mortars in 18 (axis, kind, level) groups,
-    run as 6 persistent launches) alone, and the volume RHS with it attached,
-    FP64 and mixed, with the state read from HBM (8 rotating sets)."""
+    interfaces in one element-ordered launch, 3,012 Fine2x1 mortars in 3
+    per-axis launches) alone, and the volume RHS with it attached (conforming
+    faces in the volume kernel's last phase, mortars after it), FP64 and
+    mixed, with the state read from HBM (8 rotating sets)."""
     import torch
 
     from drifter_amd import _lib, faces, synth
@@ -428,7 +429,8 @@ def measure_faces_coastline(args, stream):
         R = torch.empty_like(U)
         ms_face = time_launches(lambda: plan.face_term(U, R, _lib.LAYOUT_LINES, stream), 50, 10,
                                 stream)
-        plan.attach()
+        plan.attach()  # conforming faces in the volume kernel, launch shape tuned with them
+        s.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
         ms = time_rotating(s, U, R, 8, args.steps, args.warmup, stream)
         plan.detach()
         ng, nc, nm = plan.info()
@@ -444,9 +446,10 @@ def measure_faces_coastline(args, stream):
 
 
 def measure_faces_uniform(args, case, stream):
-    """The face term on the headline mesh (every face conforming: 3 groups,
-    765,952 interfaces) and the RHS with it, FP64 and mixed (FP32 volume and
-    surface kernels), with the state read from HBM."""
+    """The face term on the headline mesh (every face conforming: 765,952
+    interfaces) alone and the RHS with it attached (in the volume kernel's
+    last phase), FP64 and mixed (FP32 volume and surface kernels), with the
+    state read from HBM."""
     import torch
 
     from drifter_amd import _lib, faces, synth
@@ -466,10 +469,10 @@ def measure_faces_uniform(args, case, stream):
         plan = faces.FacePlan(s, b, lv, conn, kind=faces.LF_ADV, params=(0.3, -0.2, 0.1, 0.6))
         U = torch.from_numpy(synth.to_lines(case.U, E, 5)).to("cuda").to(dt)
         R = torch.empty_like(U)
-        s.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
         ms_face = time_launches(lambda: plan.face_term(U, R, _lib.LAYOUT_LINES, stream), 20, 5,
                                 stream)
-        plan.attach()
+        plan.attach()  # conforming faces in the volume kernel, launch shape tuned with them
+        s.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
         ms = time_launches(lambda: s.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream), 20, 5,
                            stream)
         plan.detach()
