@@ -12,6 +12,10 @@
 #include "lincomb.hpp"
 
 namespace drf {
+int g_rhs_per_cu[2][3][kMaxN][2] = {};
+}
+
+namespace drf {
 
 static thread_local std::string g_last_error;
 
@@ -805,6 +809,21 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // report the shape as it runs: a request above the occupancy limit runs at it
+    {
+        VolFaces<double> f64{};
+        VolFaces<float> f32{};
+        const bool fused = c->faces && (c->precision == DRF_PREC_FP64
+                                            ? volume_faces<double>(c->faces, layout, &f64)
+                                            : volume_faces<float>(c->faces, layout, &f32));
+        const int li = bl == 128 ? 0 : bl == 256 ? 1 : 2;
+        const int lim = (c->order < kMaxN)
+                            ? g_rhs_per_cu[c->precision == DRF_PREC_FP64 ? 0 : 1][li][c->order]
+                                          [fused ? 1 : 0]
+                            : 0;
+        if (lim > 0 && bw > lim)
+            bw = lim;
+    }
     c->wg_per_cu = bw;
     c->xcd_remap = br;
     c->wg_lines = bl;

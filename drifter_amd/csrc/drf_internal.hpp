@@ -126,6 +126,11 @@ template <class T> struct VolGeom {
     VolFaces<T> faces; // conforming face term added in the plain RHS (orders 1-4)
 };
 
+// Resident workgroups per CU of each plain-RHS volume kernel, by [FP64 / FP32]
+// [128 / 256 / 512 lines][order][with the fused face phase]; set by the
+// kernel's first launch (0 before), read by the autotuner (api.cpp).
+extern int g_rhs_per_cu[2][3][kMaxN][2];
+
 // Launchers, one translation unit per workgroup size (volume_fp64*.hip,
 // volume_fp32*.hip).  Return hipError_t of the launch.  ops is always given
 // in FP64; the FP32 launchers round it once on the host.

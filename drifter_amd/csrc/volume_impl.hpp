@@ -764,6 +764,9 @@ hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, Vie
     // persistent workgroups: a few rounds per workgroup so the next batch's
     // loads overlap the current batch's arithmetic
     const int wpc = (geo.wg_per_cu > 0 && geo.wg_per_cu < per_cu) ? geo.wg_per_cu : per_cu;
+    if (EPI == EPI_RHS) // what a requested shape amounts to, for the autotuner's report
+        g_rhs_per_cu[sizeof(TS) == 8 ? 0 : 1][LW == 128 ? 0 : LW == 256 ? 1 : 2][N][FACES] =
+            per_cu;
     int64_t grid = (int64_t)num_cus() * wpc;
     if (grid > nbatch)
         grid = nbatch;
