@@ -15,13 +15,15 @@
 // Work decomposition.  A workgroup owns EPB elements per batch; every element
 // is handled by n^2 threads, each owning one grid line of n nodes.  Line
 // ownership rotates xi -> eta -> zeta -> xi through LDS twice per batch, once
-// for the momentum terms and once for the tracers, so that six staged values
-// per node suffice (phases P1 .. P4b at the kernel).  Each value crosses LDS
-// once per direction instead of once per derivative tap.  Workgroups are
-// persistent: while a batch is computed, the state lines of the workgroup's
-// next batch are already being loaded into registers, so HBM traffic
-// overlaps the FP64 arithmetic; the small LDS footprint lets five N=4
-// workgroups share a CU, which hides the rest of the memory latency.
+// for the momentum terms and once for the tracers, so that seven staged
+// values per node suffice (phases P1 .. P4b at the kernel).  Each value
+// crosses LDS once per direction instead of once per derivative tap.
+// Workgroups are persistent: while a batch is computed, the state lines of
+// the workgroup's next batch are already being loaded into registers, so HBM
+// traffic overlaps the FP64 arithmetic.  A workgroup holds 128, 256 or 512
+// grid lines (at N = 4: 5, 10 or 20 elements per batch, 8 waves per CU in
+// every case, as registers and LDS allow); wider workgroups read longer
+// contiguous runs of each field plane.
 //
 // Index quirk reproduced on purpose: the free-surface tendency integrates
 // Hu, Hv over the xi index (d % n) and differentiates the column integrals
