@@ -19,6 +19,7 @@ DRF_ERR_RANGE = 5
 LAYOUT_PACKED = 0
 LAYOUT_SOA = 1
 LAYOUT_LINES = 2
+LAYOUT_BLOCKS = 3
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -46,6 +47,8 @@ SIGNATURES = {
     "drf_initialize": (_I, [_P, _L, _PD, _PD, _PD, _D, _D, _D, _PD]),
     "drf_rhs": (_I, [_P, _D, _PD, _PD]),
     "drf_rhs_dev": (_I, [_P, _D, _P, _P, _I, _P]),
+    "drf_layout_block_elements": (_I, [_I]),
+    "drf_state_len": (C.c_int64, [_P, _I]),
     "drf_set_launch": (_I, [_P, _I, _I, _I]),
     "drf_get_launch": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "drf_autotune": (_I, [_P, _P, _P, _I, _I, _P, C.POINTER(C.c_float)]),

@@ -49,7 +49,10 @@ hipStream_t pick_stream(drf_ctx *c, void *s)
 
 size_t elem_size(const drf_ctx *c) { return c->precision == DRF_PREC_FP64 ? 8 : 4; }
 
-int64_t state_len(const drf_ctx *c) { return c->E * 5 * (int64_t)c->nd; }
+int64_t state_len(const drf_ctx *c, int layout = DRF_LAYOUT_PACKED)
+{
+    return layout_state_len(layout, c->E, c->n, 5);
+}
 
 drf_status ensure_work(drf_ctx *c, size_t bytes)
 {
@@ -87,14 +90,14 @@ drf_status check_ready(const drf_ctx *c)
 template <class T> View<T> view(void *p, int layout, const drf_ctx *c)
 {
     View<T> v{(T *)p, 0, 0, 0, 0};
-    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is);
+    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is, v.bs, v.bb);
     return v;
 }
 
 template <class T> View<const T> cview(const void *p, int layout, const drf_ctx *c)
 {
     View<const T> v{(const T *)p, 0, 0, 0, 0};
-    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is);
+    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is, v.bs, v.bb);
     return v;
 }
 
@@ -107,7 +110,8 @@ template <class T> View<const T> coef_view(const T *p, const drf_ctx *c)
 
 bool valid_layout(int layout)
 {
-    return layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA || layout == DRF_LAYOUT_LINES;
+    return layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA ||
+           layout == DRF_LAYOUT_LINES || layout == DRF_LAYOUT_BLOCKS;
 }
 
 // element-major reference order h[e*nd + i + n L] -> LINES order
@@ -126,10 +130,11 @@ template <class T> std::vector<T> to_lines(const double *h, int64_t E, int n)
 // offset (element and line part); it must fit for every view of the call.
 bool lane_offsets_fit(const drf_ctx *c, int layout)
 {
-    int64_t fs, es, ls, is;
-    layout_strides(layout, c->E, c->n, 5, fs, es, ls, is);
+    View<const char> v{nullptr, 0, 0, 0, 0};
+    layout_strides(layout, c->E, c->n, 5, v.fs, v.es, v.ls, v.is, v.bs, v.bb);
     const int64_t n2 = (int64_t)c->n * c->n;
-    const int64_t max_off = ((c->E - 1) * es + (n2 - 1) * ls + 1) * (int64_t)elem_size(c);
+    const int64_t max_off =
+        (elem_offset(v, c->E - 1) + (n2 - 1) * v.ls + 1) * (int64_t)elem_size(c);
     return max_off < (int64_t(1) << 32);
 }
 
@@ -194,6 +199,8 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
 
 drf_status rhs_into(drf_ctx *c, const void *U, void *R, int layout, hipStream_t s)
 {
+    DRF_REQUIRE(!(c->faces && layout == DRF_LAYOUT_BLOCKS), DRF_ERR_UNSUPPORTED,
+                "a face plan is attached: the face kernels take the PACKED, SOA or LINES layout");
     // An attached face plan's conforming interfaces go into the volume
     // kernel's last phase where the plan allows it; its mortars follow.
     VolFaces<double> f64{};
@@ -216,7 +223,7 @@ drf_status rk_unfused(drf_ctx *c, int scheme, double dt, double *U, int layout, 
 {
     DRF_REQUIRE(c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
                 "un-fused stepping is the FP64 baseline only");
-    const int64_t len = state_len(c);
+    const int64_t len = state_len(c, layout);
     drf_status st = ensure_work(c, sizeof(double) * len * 7);
     if (st)
         return st;
@@ -344,7 +351,7 @@ drf_status rk_fused(drf_ctx *c, int scheme, double dt, void *U, int layout, hipS
             return fail(DRF_ERR_UNSUPPORTED, "device time steps need a fused scheme (not DP45)");
         return rk_unfused(c, scheme, dt, (double *)U, layout, s);
     }
-    const int64_t len = state_len(c);
+    const int64_t len = state_len(c, layout);
     const size_t es = elem_size(c);
     drf_status st = ensure_work(c, es * len * 2);
     if (st)
@@ -712,6 +719,15 @@ drf_status drf_set_launch(drf_ctx *c, int wg_per_cu, int xcd_remap, int wg_lines
     return DRF_OK;
 }
 
+int drf_layout_block_elements(int order) { return layout_block_elements(order); }
+
+int64_t drf_state_len(const drf_ctx *c, int layout)
+{
+    if (!c || c->E <= 0 || !valid_layout(layout))
+        return 0;
+    return state_len(c, layout);
+}
+
 drf_status drf_get_launch(const drf_ctx *c, int *wg_per_cu, int *xcd_remap, int *wg_lines)
 {
     DRF_REQUIRE(c && wg_per_cu && xcd_remap && wg_lines, DRF_ERR_ARG, "null argument");
@@ -744,7 +760,7 @@ drf_status drf_autotune(drf_ctx *c, const void *U, void *dUdt, int layout, int r
     // from what the previous launch left in the 256 MB last-level cache.  If
     // the copies cannot be allocated, the trials run on the caller's buffers.
     constexpr int kNset = 4;
-    const size_t bytes = elem_size(c) * (size_t)state_len(c);
+    const size_t bytes = elem_size(c) * (size_t)state_len(c, layout);
     const void *Us[kNset] = {U};
     void *Rs[kNset] = {dUdt};
     int nset = 1;
@@ -1037,7 +1053,7 @@ drf_status drf_graph_rk_adaptive(drf_ctx *c, int scheme, double cfl, double dx_m
     st = ensure_red(c, std::min<int64_t>(2048, (c->E + 3) / 4) + 1);
     if (st)
         return st;
-    return capture_graph(c, elem_size(c) * state_len(c) * 2, n_steps, out, [&](hipStream_t cs) {
+    return capture_graph(c, elem_size(c) * state_len(c, layout) * 2, n_steps, out, [&](hipStream_t cs) {
         return adaptive_step(c, scheme, cfl, dx_min, dt_min, dt_max, U, layout,
                              (double *)dt_dev, (double *)time_dev, cs);
     });
@@ -1059,8 +1075,8 @@ drf_status drf_graph_rk(drf_ctx *c, int scheme, double dt, void *U, int layout, 
     DRF_REQUIRE(fused || c->precision == DRF_PREC_FP64, DRF_ERR_UNSUPPORTED,
                 "un-fused stepping is the FP64 baseline only");
     const bool uses_lincomb = !fused || scheme == DRF_RK_DP45;
-    const size_t bytes = uses_lincomb ? sizeof(double) * state_len(c) * 7
-                                      : elem_size(c) * state_len(c) * 2;
+    const size_t bytes = uses_lincomb ? sizeof(double) * state_len(c, layout) * 7
+                                      : elem_size(c) * state_len(c, layout) * 2;
     return capture_graph(c, bytes, n_steps, out, [&](hipStream_t cs) {
         return fused ? rk_fused(c, scheme, dt, U, layout, cs)
                      : rk_unfused(c, scheme, dt, (double *)U, layout, cs);

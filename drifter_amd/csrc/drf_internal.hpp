@@ -41,20 +41,56 @@ struct Phys {
 // Strided view of a multi-field nodal state.  Node d = i + n L of element e
 // (i the xi index, L = j + n k the grid line) in field f lives at
 // p[f * fs + e * es + L * ls + i * is].
+// A field array in one of the public layouts: value (field f, element e,
+// line L, xi index i) at f fs + elem(e) + L ls + i is, elem(e) = e es, or in
+// the blocked layout (bb > 0) (e / bb) bs + (e % bb) es.
 template <class T> struct View {
     T *p;
     int64_t fs, es, ls, is;
+    int64_t bs = 0; // stride between blocks (blocked layout)
+    int64_t bb = 0; // elements per block, 0: not blocked
 };
+
+// Volume-kernel batch geometry, shared by the kernels and the host: seven
+// LDS slots per element node plus the double-buffered column integrals, and
+// the LDS a workgroup may give its elements (gfx950: 160 KB per workgroup,
+// less room for the staged operators).
+constexpr int kVolSlots = 7;
+constexpr int kVolLdsBudget = 160 * 1024 - 1024;
+// elements per batch of a workgroup of `lines` grid lines at `order`
+constexpr int vol_batch_elements(int order, int lines)
+{
+    const int n = order + 1, n2 = n * n, n3 = n2 * n;
+    const int lds = (kVolSlots * n3 + 4 * n2) * 8;
+    const int by_lines = lines / n2 > 0 ? lines / n2 : 1;
+    const int by_lds = kVolLdsBudget / lds > 0 ? kVolLdsBudget / lds : 1;
+    return by_lines < by_lds ? by_lines : by_lds;
+}
+// elements per block of DRF_LAYOUT_BLOCKS: the batch of the widest workgroup
+// (512 lines up to order 4, 128 above), which every narrower batch divides
+inline int layout_block_elements(int order)
+{
+    return (order >= 1 && order <= 8) ? vol_batch_elements(order, order <= 4 ? 512 : 128) : 0;
+}
 
 // Strides of the public layouts (drf.h).  LINES is the kernels' native
 // layout: for a fixed xi index, consecutive (element, line) pairs are
 // consecutive in memory, so a wavefront whose lanes own consecutive lines
 // touches one contiguous span per load or store.
 inline void layout_strides(int layout, int64_t E, int n, int nfields, int64_t &fs, int64_t &es,
-                           int64_t &ls, int64_t &is)
+                           int64_t &ls, int64_t &is, int64_t &bs, int64_t &bb)
 {
     const int64_t nd = (int64_t)n * n * n, n2 = (int64_t)n * n;
-    if (layout == DRF_LAYOUT_LINES) {
+    bs = 0;
+    bb = 0;
+    if (layout == DRF_LAYOUT_BLOCKS) {
+        bb = layout_block_elements(n - 1);
+        es = n2;
+        ls = 1;
+        is = bb * n2;
+        fs = (int64_t)n * is;
+        bs = (int64_t)nfields * fs;
+    } else if (layout == DRF_LAYOUT_LINES) {
         fs = E * nd;
         es = n2;
         ls = 1;
@@ -72,10 +108,32 @@ inline void layout_strides(int layout, int64_t E, int n, int nfields, int64_t &f
     }
 }
 
+// the element's part of an offset (every layout)
+template <class T> __host__ __device__ inline int64_t elem_offset(const View<T> &v, int64_t e)
+{
+    if (v.bb) {
+        // 32-bit division where the element index allows it (every mesh here)
+        const int64_t blk = (e >> 31) ? e / v.bb : (int64_t)((uint32_t)e / (uint32_t)v.bb);
+        return blk * v.bs + (e - blk * v.bb) * v.es;
+    }
+    return e * v.es;
+}
+
 template <class T> __host__ __device__ inline int64_t node_offset(const View<T> &v, int64_t e, int L,
                                                                   int i)
 {
-    return e * v.es + (int64_t)L * v.ls + (int64_t)i * v.is;
+    return elem_offset(v, e) + (int64_t)L * v.ls + (int64_t)i * v.is;
+}
+
+// values of one state of E elements in a layout (whole blocks when blocked)
+inline int64_t layout_state_len(int layout, int64_t E, int n, int nfields)
+{
+    const int64_t nd = (int64_t)n * n * n;
+    if (layout == DRF_LAYOUT_BLOCKS) {
+        const int64_t B = layout_block_elements(n - 1);
+        return B > 0 ? (E + B - 1) / B * B * nfields * nd : 0;
+    }
+    return E * nfields * nd;
 }
 
 // Epilogue of the volume kernel: what is done with the tendency k of each node.

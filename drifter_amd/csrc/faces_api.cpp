@@ -135,7 +135,8 @@ template <class T> void fill_args(const drf_face_plan *p, FaceArgs<T> &a, const 
     std::memset(&a, 0, sizeof(a));
     a.U = (const T *)U;
     a.dU = (T *)dU;
-    layout_strides(layout, p->E, p->n, 5, a.fs, a.es, a.ls, a.is);
+    int64_t bs = 0, bb = 0; // the face entry points refuse the blocked layout
+    layout_strides(layout, p->E, p->n, 5, a.fs, a.es, a.ls, a.is, bs, bb);
     double x[kMaxN], w[kMaxN];
     lgl_nodes(p->n, x, w);
     for (int i = 0; i < p->n; ++i)
@@ -494,6 +495,8 @@ drf_status drf_face_plan_lists(const drf_face_plan *p, int64_t *conforming, int6
 drf_status drf_face_term_dev(drf_face_plan *p, const void *U, void *dU, int layout, void *stream)
 {
     DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_term_dev: null plan");
+    DRF_REQUIRE(layout != DRF_LAYOUT_BLOCKS, DRF_ERR_UNSUPPORTED,
+                "drf_face_term_dev: the face kernels take the PACKED, SOA or LINES layout");
     DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA ||
                     layout == DRF_LAYOUT_LINES,
                 DRF_ERR_ARG, "drf_face_term_dev: unknown layout");
@@ -508,6 +511,8 @@ drf_status drf_face_group_fluxes_dev(drf_face_plan *p, int group, const void *U,
     DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_group_fluxes_dev: null plan");
     DRF_REQUIRE(group >= 0 && group < (int)p->groups.size(), DRF_ERR_RANGE,
                 "drf_face_group_fluxes_dev: group out of range");
+    DRF_REQUIRE(layout != DRF_LAYOUT_BLOCKS, DRF_ERR_UNSUPPORTED,
+                "drf_face_group_fluxes_dev: the face kernels take the PACKED, SOA or LINES layout");
     DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA ||
                     layout == DRF_LAYOUT_LINES,
                 DRF_ERR_ARG, "drf_face_group_fluxes_dev: unknown layout");

@@ -65,7 +65,8 @@ extern "C" drf_status drf_max_wave_speed_noalloc_dev(drf_ctx *c, const void *U, 
         return fail(DRF_ERR_STATE, "drf_max_wave_speed_noalloc_dev: context not initialized");
     if (!U || !c_max)
         return fail(DRF_ERR_ARG, "drf_max_wave_speed_noalloc_dev: null argument");
-    if (layout != DRF_LAYOUT_PACKED && layout != DRF_LAYOUT_SOA && layout != DRF_LAYOUT_LINES)
+    if (layout != DRF_LAYOUT_PACKED && layout != DRF_LAYOUT_SOA && layout != DRF_LAYOUT_LINES &&
+        layout != DRF_LAYOUT_BLOCKS)
         return fail(DRF_ERR_ARG, "drf_max_wave_speed_noalloc_dev: unknown layout");
     if (c->precision != DRF_PREC_FP64)
         return fail(DRF_ERR_UNSUPPORTED,
@@ -81,12 +82,10 @@ extern "C" drf_status drf_max_wave_speed_noalloc_dev(drf_ctx *c, const void *U, 
         return fail(DRF_ERR_STATE,
                     "drf_max_wave_speed_noalloc_dev: scratch not registered on this device");
     hipError_t e = hipSetDevice(c->device);
-    int64_t fs, es, ls, is;
-    layout_strides(layout, c->E, c->n, 5, fs, es, ls, is);
-    const View<const double> Uv{(const double *)U, fs, es, ls, is};
-    int64_t hfs, hes, hls, his;
-    layout_strides(DRF_LAYOUT_LINES, c->E, c->n, 5, hfs, hes, hls, his); // bathymetry: LINES
-    const View<const double> hv{c->d_h, hfs, hes, hls, his};
+    View<const double> Uv{(const double *)U, 0, 0, 0, 0};
+    layout_strides(layout, c->E, c->n, 5, Uv.fs, Uv.es, Uv.ls, Uv.is, Uv.bs, Uv.bb);
+    View<const double> hv{c->d_h, 0, 0, 0, 0}; // bathymetry: LINES
+    layout_strides(DRF_LAYOUT_LINES, c->E, c->n, 5, hv.fs, hv.es, hv.ls, hv.is, hv.bs, hv.bb);
     const hipStream_t s = (hipStream_t)stream; // as given; NULL is HIP's null stream
     if (e == hipSuccess)
         e = launch_wave_speed(c->order, c->E, Uv, hv, part, npart, s);

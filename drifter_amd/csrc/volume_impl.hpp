@@ -42,19 +42,14 @@ namespace vol {
 
 // LDS slots per element node, used in two rounds (see the kernel).
 enum : int { S_A = 0, S_B, S_C, S_D, S_E, S_F, S_G, S_NF };
-
-// LDS a workgroup may take for its elements (gfx950: 160 KB per workgroup),
-// leaving room for the staged D.
-constexpr int kLdsBudget = 160 * 1024 - 1024;
+static_assert(S_NF == kVolSlots, "the batch geometry counts the kernel's LDS slots");
 
 // LW: grid lines per workgroup (128, 256 or 512), the workgroup size; the
-// elements per batch EPB follow from it and from the LDS budget.
+// elements per batch EPB follow from it and from the LDS budget
+// (vol_batch_elements, shared with the blocked layout's block size).
 template <int N, int LW> struct Cfg {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
-    static constexpr int lds_per_elem_fp64 = (S_NF * n3 + 4 * n2) * 8; // slots + column integrals
-    static constexpr int epb_thr = (LW / n2) > 0 ? LW / n2 : 1;
-    static constexpr int epb_lds = (kLdsBudget / lds_per_elem_fp64) > 0 ? kLdsBudget / lds_per_elem_fp64 : 1;
-    static constexpr int EPB = epb_thr < epb_lds ? epb_thr : epb_lds;
+    static constexpr int EPB = vol_batch_elements(N, LW);
     static constexpr int NT = EPB * n2;
     static constexpr int BLOCK = ((NT + 63) / 64) * 64;
     // minimum waves per SIMD the register allocation must allow
@@ -99,9 +94,30 @@ __device__ __forceinline__ void sto(TS *ubase, uint32_t off, TS v)
     else
         *(TS *)((char *)ubase + off) = v;
 }
-template <class T> __device__ __forceinline__ uint32_t lane_off(const View<T> &v, int64_t e, int L)
+// Byte offsets of (element, line) pairs.  32-bit arithmetic is exact: the
+// host checks that every offset of the launch fits in 32 bits.
+//
+// A view without blocks: the bathymetry and Coriolis lines (always in the
+// line layout), and the fused face phase (face plans refuse the blocked
+// layout).
+template <class T> __device__ __forceinline__ uint32_t flat_off(const View<T> &v, int64_t e, int L)
 {
-    return (uint32_t)((e * v.es + (int64_t)L * v.ls) * (int64_t)sizeof(T));
+    return ((uint32_t)e * (uint32_t)v.es + (uint32_t)L * (uint32_t)v.ls) * (uint32_t)sizeof(T);
+}
+// Element e of batch b in a state view.  All state views of one launch share
+// the layout and its strides.  In the blocked layout a batch never straddles
+// a block (a block's element count is a multiple of the batch's, checked at
+// launch), so the block is b / (bb / EPB): a division of uniform values, once
+// per batch, none per lane.  Lanes that repeat element E - 1 sit in the last
+// batch, in that element's block.  Without blocks (bb = 0) the block is 0.
+template <int EPB, class T>
+__device__ __forceinline__ uint32_t state_off(const View<T> &v, int64_t b, int64_t e, int L)
+{
+    const uint32_t q = v.bb ? (uint32_t)v.bb / (uint32_t)EPB : 0xffffffffu;
+    const uint32_t blk = (uint32_t)b / q;
+    const uint32_t eb = (uint32_t)e - blk * (uint32_t)v.bb;
+    return (blk * (uint32_t)v.bs + eb * (uint32_t)v.es + (uint32_t)L * (uint32_t)v.ls) *
+           (uint32_t)sizeof(T);
 }
 
 template <class TC> __device__ __forceinline__ TC hinv_of(TC H)
@@ -151,11 +167,11 @@ template <int n, class TC> struct Lines {
 // them: a load issued inside P1 would make the wave wait for every store of
 // the previous batch as well (gfx9 counts loads and stores in one in-order
 // counter).
-template <int n, class TS, class TC>
+template <int n, int EPB, class TS, class TC>
 __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS> &U,
-                                           const VolGeom<TS> &geo, int64_t e, int line)
+                                           const VolGeom<TS> &geo, int64_t b, int64_t e, int line)
 {
-    const uint32_t uo = lane_off(U, e, line), ho = lane_off(geo.h, e, line);
+    const uint32_t uo = state_off<EPB>(U, b, e, line), ho = flat_off(geo.h, e, line);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
         L.hu[i] = ldo<TS, TC>(U.p + 0 * U.fs + i * U.is, uo);
@@ -166,7 +182,7 @@ __device__ __forceinline__ void load_lines(Lines<n, TC> &L, const View<const TS>
         L.hb[i] = ldo<TS, TC>(geo.h.p + i * geo.h.is, ho);
     }
     if (geo.f.p) {
-        const uint32_t fo = lane_off(geo.f, e, line);
+        const uint32_t fo = flat_off(geo.f, e, line);
 #pragma unroll
         for (int i = 0; i < n; ++i)
             L.fc[i] = ldo<TS, TC>(geo.f.p + i * geo.f.is, fo);
@@ -181,9 +197,10 @@ constexpr int kXcds = 8; // MI355X: 8 XCDs of 32 CUs
 
 // Writes fields [F0, F0 + NFD) of the thread's xi-line: the tendency k (RHS)
 // or a fused Runge-Kutta stage update, whose operand lines are loaded here,
-// field by field, right before they are combined.
+// field by field, right before they are combined.  so: the line's byte
+// offset in every state view (state_off).
 template <int n, int F0, int NFD, class TS, class TC, int EPI>
-__device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int L,
+__device__ __forceinline__ void write_fields(const Epi<TS> &epi, uint32_t so,
                                              const TC (&kv)[NFD][n])
 {
     double d1 = epi.c1, d2 = epi.c2;
@@ -196,7 +213,7 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
     }
     const TC c0 = (TC)epi.c0, c1 = (TC)d1, c2 = (TC)d2, c3 = (TC)epi.c3;
     if (EPI == EPI_RHS) {
-        const uint32_t ro = lane_off(epi.R, e, L);
+        const uint32_t ro = so;
 #pragma unroll
         for (int f = 0; f < NFD; ++f)
 #pragma unroll
@@ -207,13 +224,11 @@ __device__ __forceinline__ void write_fields(const Epi<TS> &epi, int64_t e, int 
     const View<const TS> V1 = (EPI == EPI_LSRK) ? epi.X : epi.U0;
     const View<const TS> V2 = (EPI == EPI_COMB)
                                   ? epi.X
-                                  : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls, epi.A.is};
+                                  : View<const TS>{epi.A.p, epi.A.fs, epi.A.es, epi.A.ls,
+                                                   epi.A.is, epi.A.bs, epi.A.bb};
     const bool two = (EPI == EPI_STAGE_ACC || EPI == EPI_RK4FIN || EPI == EPI_COMB ||
                       (EPI == EPI_LSRK && epi.acc_mode != 1));
-    const uint32_t q1 = lane_off(V1, e, L);
-    const uint32_t q2 = two ? lane_off(V2, e, L) : 0u;
-    const uint32_t yo = lane_off(epi.Y, e, L);
-    const uint32_t ao = epi.A.p ? lane_off(epi.A, e, L) : 0u;
+    const uint32_t q1 = so, q2 = so, yo = so, ao = so;
 #pragma unroll
     for (int f = 0; f < NFD; ++f) {
         TC o1[n], o2[n];
@@ -288,11 +303,11 @@ __device__ __forceinline__ void face_phase(const VolFaces<TS> &fc, const View<co
         Nb r;
         r.on = v >= 0;
         r.side = (int)(v & 1);
-        r.off = r.on ? lane_off(U, v >> 4, line) : 0u;
+        r.off = r.on ? flat_off(U, v >> 4, line) : 0u;
         return r;
     };
     const Nb x0 = entry(0, L), x1 = entry(1, L), y = entry(fy, Ly), z = entry(fz, Lz);
-    const uint32_t ou = lane_off(U, e, L), orr = lane_off(R, e, L);
+    const uint32_t ou = flat_off(U, e, L), orr = flat_off(R, e, L);
 #pragma unroll 1
     for (int fld = 0; fld < 5; ++fld) {
         const TS *Uf = U.p + fld * U.fs;
@@ -430,7 +445,7 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
     if (PREFETCH) {
         const int64_t e0 = elem(b);
         if (in_tile && b < nbatch && e0 < geo.E)
-            load_lines<n, TS, TC>(nxt, U, geo, e0, L);
+            load_lines<n, EPB, TS, TC>(nxt, U, geo, b, e0, L);
     }
 
     for (int par = 0; b < nbatch; b += G, par ^= 1) {
@@ -441,7 +456,7 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
         // loop carries no register copies that would wait for those loads.
         Lines<n, TC> own;
         if (!PREFETCH && act)
-            load_lines<n, TS, TC>(own, U, geo, e, L);
+            load_lines<n, EPB, TS, TC>(own, U, geo, b, e, L);
         const Lines<n, TC> &cur = PREFETCH ? nxt : own;
         TC *s = &sm[le][0][0];
 
@@ -521,7 +536,7 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
             const int64_t bn = b + G;
             const int64_t en = elem(bn);
             if (in_tile && bn < nbatch && en < geo.E)
-                load_lines<n, TS, TC>(nxt, U, geo, en, L);
+                load_lines<n, EPB, TS, TC>(nxt, U, geo, bn, en, L);
         }
 
         // ---------------- P2a: eta-line (i, k) = (L % n, L / n) ----------------
@@ -601,7 +616,7 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
                 kv[0][i] = pHu[i] + s[S_A * n3 + m];
                 kv[1][i] = pHv[i] + s[S_B * n3 + m];
             }
-            write_fields<n, 0, 2, TS, TC, EPI>(epi, e, L, kv);
+            write_fields<n, 0, 2, TS, TC, EPI>(epi, state_off<EPB>(U, b, e, L), kv);
             TC u[n], v[n], hi[n];
 #pragma unroll
             for (int i = 0; i < n; ++i) {
@@ -699,7 +714,7 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
                 kv[1][i] = (pHT[i] + s[S_A * n3 + m]) + s[S_C * n3 + m];
                 kv[2][i] = (pHS[i] + s[S_B * n3 + m]) + s[S_D * n3 + m];
             }
-            write_fields<n, 2, 3, TS, TC, EPI>(epi, e, L, kv);
+            write_fields<n, 2, 3, TS, TC, EPI>(epi, state_off<EPB>(U, b, e, L), kv);
         }
         // ---------------- P5 (FACES): the conforming face term ----------------
         // Not by the lanes that repeat element E - 1 from another wavefront:
@@ -752,6 +767,9 @@ hipError_t launch_ne(const VolGeom<TS> &geo, const Ops &ops, const Phys &ph, Vie
     using C = Cfg<N, LW>;
     if (geo.E <= 0)
         return hipSuccess;
+    // blocked states: whole batches per block; the face phase reads unblocked states only
+    if (U.bb && (U.bb % C::EPB != 0 || FACES))
+        return hipErrorInvalidValue;
     const int64_t nbatch = (geo.E + C::EPB - 1) / C::EPB;
     auto kern = volume_kernel<N, TS, TC, EPI, LW, FACES>;
     // resident workgroups per CU (queried once per instantiation: the launch

@@ -107,17 +107,20 @@ class PrimitiveEquationsSolver:
     def ctx(self):
         return self._ctx
 
-    def state_len(self):
-        """Values in one packed / SoA / line-layout state: E * 5 * ndof."""
-        return self.num_elements * 5 * self.ndof
+    def state_len(self, layout=None):
+        """Values in one state: E * 5 * ndof in the packed, SoA and line
+        layouts; whole blocks in the blocked layout (drf_state_len)."""
+        if layout is None or layout != _lib.LAYOUT_BLOCKS:
+            return self.num_elements * 5 * self.ndof
+        return int(self._l.drf_state_len(self._ctx, layout))
 
     def _state_dtype(self):
         import torch
 
         return torch.float64 if self.precision == _lib.PREC_FP64 else torch.float32
 
-    def _dev_state(self, t, name="U"):
-        return _dev_array(t, self.state_len(), self._state_dtype(), name)
+    def _dev_state(self, t, name="U", layout=None):
+        return _dev_array(t, self.state_len(layout), self._state_dtype(), name)
 
     def _dev_scalar(self, t, name):
         import torch
@@ -173,14 +176,15 @@ class PrimitiveEquationsSolver:
         """Time every launch shape on (U, R), keep the fastest; returns
         (wg_per_cu, xcd_remap, wg_lines, microseconds per launch)."""
         us = C.c_float()
-        check(self._l.drf_autotune(self._ctx, self._dev_state(U), self._dev_state(R, "dUdt"),
-                                   layout, reps, _sptr(stream), C.byref(us)))
+        check(self._l.drf_autotune(self._ctx, self._dev_state(U, "U", layout),
+                                   self._dev_state(R, "dUdt", layout), layout, reps, _sptr(stream),
+                                   C.byref(us)))
         w, r, n = self.get_launch()
         return w, r, n, us.value
 
     def compute_rhs_dev(self, U, dUdt, layout=_lib.LAYOUT_SOA, stream=None, t=0.0):
-        check(self._l.drf_rhs_dev(self._ctx, t, self._dev_state(U), self._dev_state(dUdt, "dUdt"),
-                                  layout, _sptr(stream)))
+        check(self._l.drf_rhs_dev(self._ctx, t, self._dev_state(U, "U", layout),
+                                  self._dev_state(dUdt, "dUdt", layout), layout, _sptr(stream)))
 
     def rk_step(self, scheme, t, dt, U):
         U = np.array(_host_state(U, self.state_len()), copy=True)
@@ -188,14 +192,14 @@ class PrimitiveEquationsSolver:
         return U
 
     def rk_step_dev(self, scheme, t, dt, U, layout=_lib.LAYOUT_SOA, fused=True, stream=None):
-        check(self._l.drf_rk_step_dev(self._ctx, scheme, t, dt, self._dev_state(U), layout,
-                                      int(fused), _sptr(stream)))
+        check(self._l.drf_rk_step_dev(self._ctx, scheme, t, dt, self._dev_state(U, "U", layout),
+                                      layout, int(fused), _sptr(stream)))
 
     def capture_rk_steps(self, scheme, dt, U, layout=_lib.LAYOUT_SOA, fused=True, n_steps=1):
         """StepGraph of n_steps drf_rk_step_dev calls on U (in place)."""
         h = C.c_void_p()
-        check(self._l.drf_graph_rk(self._ctx, scheme, dt, self._dev_state(U), layout, int(fused),
-                                   n_steps, C.byref(h)))
+        check(self._l.drf_graph_rk(self._ctx, scheme, dt, self._dev_state(U, "U", layout), layout,
+                                   int(fused), n_steps, C.byref(h)))
         return StepGraph(h)
 
     def rk_step_adaptive_dev(self, scheme, U, dt_dev, time_dev=None, cfl=0.5, dt_min=1e-6,
@@ -205,7 +209,7 @@ class PrimitiveEquationsSolver:
         step.  dx_min defaults to the driver's 2 / (N + 1)."""
         dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
         check(self._l.drf_rk_step_adaptive_dev(self._ctx, scheme, cfl, dx, dt_min, dt_max,
-                                               self._dev_state(U), layout,
+                                               self._dev_state(U, "U", layout), layout,
                                                self._dev_scalar(dt_dev, "dt_dev"),
                                                self._dev_scalar(time_dev, "time_dev"),
                                                _sptr(stream)))
@@ -216,7 +220,7 @@ class PrimitiveEquationsSolver:
         dx = 2.0 / (self.order + 1) if dx_min is None else dx_min
         h = C.c_void_p()
         check(self._l.drf_graph_rk_adaptive(self._ctx, scheme, cfl, dx, dt_min, dt_max,
-                                            self._dev_state(U), layout, n_steps,
+                                            self._dev_state(U, "U", layout), layout, n_steps,
                                             self._dev_scalar(dt_dev, "dt_dev"),
                                             self._dev_scalar(time_dev, "time_dev"), C.byref(h)))
         return StepGraph(h)
@@ -229,8 +233,8 @@ class PrimitiveEquationsSolver:
 
     def max_wave_speed_dev(self, U, layout=_lib.LAYOUT_SOA, stream=None):
         c = C.c_double()
-        check(self._l.drf_max_wave_speed_dev(self._ctx, self._dev_state(U), layout, C.byref(c),
-                                             _sptr(stream)))
+        check(self._l.drf_max_wave_speed_dev(self._ctx, self._dev_state(U, "U", layout), layout,
+                                             C.byref(c), _sptr(stream)))
         return c.value
 
     def max_wave_speed_noalloc_dev(self, U, c_max, layout=_lib.LAYOUT_SOA, stream=None):
@@ -241,8 +245,8 @@ class PrimitiveEquationsSolver:
         import torch
 
         check(self._l.drf_max_wave_speed_noalloc_dev(
-            self._ctx, self._dev_state(U), layout, _dev_array(c_max, 1, torch.float64, "c_max"),
-            _sptr(stream)))
+            self._ctx, self._dev_state(U, "U", layout), layout,
+            _dev_array(c_max, 1, torch.float64, "c_max"), _sptr(stream)))
 
     def max_wave_speed_derived_dev(self, H, u, v, elem_stride, stream=None):
         import torch

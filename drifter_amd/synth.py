@@ -285,6 +285,21 @@ def from_lines(V, E, n):
     return np.ascontiguousarray(P.transpose(2, 0, 3, 1).reshape(-1))
 
 
+def to_blocks(U, E, n, B):
+    """packed [e][f][i + n L] -> blocked [e // B][f][i][e % B][L], the last
+    block padded with zeros to B elements"""
+    nb = -(-E // B)
+    P = np.zeros((nb * B, 5, n * n, n))
+    P[:E] = U.reshape(E, 5, n * n, n)  # e, f, L, i
+    return np.ascontiguousarray(P.reshape(nb, B, 5, n * n, n).transpose(0, 2, 4, 1, 3).reshape(-1))
+
+
+def from_blocks(V, E, n, B):
+    nb = -(-E // B)
+    P = V.reshape(nb, 5, n, B, n * n)  # block, f, i, e, L
+    return np.ascontiguousarray(P.transpose(0, 3, 1, 4, 2).reshape(nb * B, -1)[:E].reshape(-1))
+
+
 def to_soa(U, E, nd):
     return np.ascontiguousarray(U.reshape(E, 5, nd).transpose(1, 0, 2).reshape(-1))
 

@@ -14,6 +14,16 @@
  *   DRF_LAYOUT_LINES   U[field][i][e][L] with d = i + n L -- structure of
  *                      arrays with the xi index outermost: the kernels'
  *                      native layout, fully coalesced loads and stores
+ *   DRF_LAYOUT_BLOCKS  U[block][field][i][e % B][L], block = e / B, with
+ *                      B = drf_layout_block_elements(order): the LINES
+ *                      layout cut into blocks of B elements, so that a
+ *                      workgroup's batch reads and writes one contiguous
+ *                      region; the last block is allocated whole
+ *                      (drf_state_len) and its slots past element E - 1
+ *                      are neither read nor written.  The volume kernel
+ *                      addresses a whole state in 32-bit byte offsets here
+ *                      (under 4 GiB per state, DRF_ERR_UNSUPPORTED above);
+ *                      a face plan attached to the context refuses it.
  *   fields: 0 Hu, 1 Hv, 2 eta, 3 HT, 4 HS
  *   full state (mode splitting): [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d]
  *
@@ -48,6 +58,7 @@ typedef int drf_status;
 #define DRF_LAYOUT_PACKED 0
 #define DRF_LAYOUT_SOA 1
 #define DRF_LAYOUT_LINES 2
+#define DRF_LAYOUT_BLOCKS 3
 
 #define DRF_PREC_FP64 0  /* FP64 storage and arithmetic (parity path) */
 #define DRF_PREC_MIXED 1 /* FP32 volume arithmetic and storage; FP64 column integrals */
@@ -105,6 +116,15 @@ drf_status drf_initialize(drf_ctx *ctx, int64_t n_elem, const double *h, const d
  * and dUdt of length E*5*ndof, as SimulationDriver::compute_rhs
  * (src/solver/simulation_driver.cpp:170-197) produces it. */
 drf_status drf_rhs(drf_ctx *ctx, double t, const double *U, double *dUdt);
+
+/* Elements per block of DRF_LAYOUT_BLOCKS at an order (no counterpart in the
+ * reference): the volume kernel's batch at its widest workgroup, so that
+ * every batch lies in one block.  0 for an order outside 1-8. */
+int drf_layout_block_elements(int order);
+/* Values of one state (5 fields) of the context's mesh in `layout`:
+ * E*5*(N+1)^3, and for DRF_LAYOUT_BLOCKS whole blocks (E rounded up to a
+ * multiple of the block).  0 before drf_initialize or for an unknown layout. */
+int64_t drf_state_len(const drf_ctx *ctx, int layout);
 
 /* Device-resident variant: U and dUdt are device pointers in `layout`,
  * element type double (FP64 context) or float (mixed context). */

@@ -87,6 +87,33 @@ def test_basis_export():
     assert lib.drf_basis(0, None, None, None) == _lib.DRF_ERR_ARG
 
 
+def test_blocked_layout_block_sizes_and_host_conversion():
+    """Elements per block of the blocked layout: the volume kernel's widest
+    batch (512 lines up to order 4, 128 above, at most 20 elements of order 4
+    by the LDS budget); every narrower workgroup's batch divides it.  The
+    host conversion round-trips and puts node (e, f, i, L) at
+    ((((e / B) 5 + f) n + i) B + e % B) n^2 + L."""
+    from drifter_amd import synth
+
+    lib = _lib.load()
+    got = [lib.drf_layout_block_elements(o) for o in range(0, 10)]
+    assert got == [0, 128, 56, 32, 20, 3, 2, 2, 1, 0]
+    for order, B in zip(range(1, 5), got[1:5]):
+        n2 = (order + 1) ** 2
+        for lines in (128, 256):
+            assert B % max(1, min(lines // n2, B)) == 0
+    rng = np.random.default_rng(0)
+    for order in (1, 4, 5, 8):
+        n, B, E = order + 1, got[order], 455
+        U = rng.random(E * 5 * n ** 3)
+        V = synth.to_blocks(U, E, n, B)
+        assert V.size == -(-E // B) * B * 5 * n ** 3
+        assert np.array_equal(synth.from_blocks(V, E, n, B), U)
+        for e, f, i, L in ((0, 0, 0, 0), (E - 1, 4, n - 1, n * n - 1), (B, 2, 1, 3)):
+            off = (((e // B * 5 + f) * n + i) * B + e % B) * n * n + L
+            assert V[off] == U[(e * 5 + f) * n ** 3 + i + n * L]
+
+
 def test_missing_library_fails_loudly(tmp_path):
     """No CPU fallback: without the built HIP library the package raises
     instead of computing anything (checked in a fresh interpreter so the
