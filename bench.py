@@ -89,6 +89,38 @@ def rel_l2_fields(a, b, E, nd):
     return out
 
 
+def head_layout(args):
+    from drifter_amd import _lib
+
+    return _lib.LAYOUT_BLOCKS if args.layout == "blocks" else _lib.LAYOUT_LINES
+
+
+def state_in(args, case, dev):
+    """The case's FP64 state on the device in the headline layout."""
+    import torch
+
+    from drifter_amd import _lib, synth
+
+    n = case.order + 1
+    if args.layout == "blocks":
+        B = _lib.load().drf_layout_block_elements(case.order)
+        host = synth.to_blocks(case.U, case.E, n, B)
+    else:
+        host = synth.to_lines(case.U, case.E, n)
+    return torch.from_numpy(host).to(dev)
+
+
+def per_field(x, args, case):
+    """(5, m) field-major view of a host state in the headline layout (the
+    blocked layout's padding included: zero in the inputs, never written)."""
+    from drifter_amd import _lib
+
+    if args.layout == "blocks":
+        blk = 5 * (case.order + 1) ** 3 * _lib.load().drf_layout_block_elements(case.order)
+        return x.reshape(-1, 5, blk // 5).transpose(1, 0, 2).reshape(5, -1)
+    return x.reshape(5, -1)
+
+
 def _fmt(x):
     return None if x is None else float("%.3e" % x)
 
@@ -228,28 +260,29 @@ def measure_mixed_and_steps(args, case, s64, U, r64, stream, label):
 
     E, nd = case.E, case.nd
     dof = E * nd
+    lay = head_layout(args)
     s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
     U32 = U.to(torch.float32)
     R32 = torch.empty_like(U32)
-    tune32 = s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
-    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
+    tune32 = s32.autotune(U32, R32, lay, reps=4, stream=stream)
+    ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, lay, stream),
                          args.steps, args.warmup, stream)
-    err32 = rel_l2_fields(R32.double().cpu().numpy(), r64, E, nd)
+    err32 = rel_l2_fields(per_field(R32.double().cpu().numpy(), args, case), r64, E, nd)
     del R32
     log(f"mixed RHS ({label}): {ms32 * 1e3:.1f} us, {dof * 5 / (ms32 * 1e-3) / 1e9:.1f} GDOF/s, "
         f"rel L2 vs FP64 {err32}")
 
-    c = s64.max_wave_speed_dev(U, _lib.LAYOUT_LINES, stream)
+    c = s64.max_wave_speed_dev(U, lay, stream)
     dt = AdaptiveTimeController(0.5, 1e-6, 3600.0).compute_dt(c, 2.0 / 5)
     Uw = U.clone()
-    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
+    ms_rk_f = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, lay,
                                                     True, stream), 10, 2, stream)
     Uw.copy_(U)
-    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, _lib.LAYOUT_LINES,
+    ms_rk_u = time_launches(lambda: s64.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw, lay,
                                                     False, stream), 10, 2, stream)
     Uw32 = U32.clone()
-    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, _lib.LAYOUT_LINES,
+    ms_rk_m = time_launches(lambda: s32.rk_step_dev(_lib.RK_RK4, 0.0, dt, Uw32, lay,
                                                     True, stream), 10, 2, stream)
     del Uw32
     log(f"RK4 step ({label}) FP64 fused {ms_rk_f:.3f} ms, un-fused {ms_rk_u:.3f} ms, mixed fused "
@@ -267,15 +300,15 @@ def measure_mixed_and_steps(args, case, s64, U, r64, stream, label):
         Ub = U32.clone()
         Uf = Ur.clone()
         for k in range(1, max(marks) + 1):
-            s64.rk_step_dev(_lib.RK_RK4, 0.0, step, Uw, _lib.LAYOUT_LINES, True, stream)
-            s32.rk_step_dev(_lib.RK_RK4, 0.0, step, Ub, _lib.LAYOUT_LINES, True, stream)
-            s64.rk_step_dev(_lib.RK_RK4, 0.0, step, Uf, _lib.LAYOUT_LINES, True, stream)
+            s64.rk_step_dev(_lib.RK_RK4, 0.0, step, Uw, lay, True, stream)
+            s32.rk_step_dev(_lib.RK_RK4, 0.0, step, Ub, lay, True, stream)
+            s64.rk_step_dev(_lib.RK_RK4, 0.0, step, Uf, lay, True, stream)
             if k in marks:
-                ref = Uw.cpu().numpy()
+                ref = per_field(Uw.cpu().numpy(), args, case)
                 key = f"{k}x{label_k}"
                 steps_err[key] = [_fmt(x) for x in
-                                  rel_l2_fields(Ub.double().cpu().numpy(), ref, E, nd)]
-                floor_err[key] = [_fmt(x) for x in rel_l2_fields(Uf.cpu().numpy(), ref, E, nd)]
+                                  rel_l2_fields(per_field(Ub.double().cpu().numpy(), args, case), ref, E, nd)]
+                floor_err[key] = [_fmt(x) for x in rel_l2_fields(per_field(Uf.cpu().numpy(), args, case), ref, E, nd)]
         del Ub, Uf
     log(f"mixed vs FP64 rel L2 (dt {dt:.4g}): {steps_err}; FP64 from the FP32-rounded state: "
         f"{floor_err}")
@@ -777,6 +810,10 @@ def main():
                     help="fixed volume-kernel launch shape (workgroups per CU, XCD order 0/1, "
                          "grid lines per workgroup 128/256/512) instead of the autotuner's "
                          "choice")
+    ap.add_argument("--layout", choices=("blocks", "lines"), default="blocks",
+                    help="state layout in HBM for the headline mesh: blocked xi-planes "
+                         "(DRF_LAYOUT_BLOCKS, default) or whole-mesh xi-planes "
+                         "(DRF_LAYOUT_LINES)")
     ap.add_argument("--rhs-only", action="store_true",
                     help="time the FP64 RHS only (no mixed-precision or RK extras)")
     ap.add_argument("--profile-mixed", action="store_true",
@@ -837,13 +874,13 @@ def main():
     if args.profile_mixed:
         s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
         s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-        U32 = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev).to(torch.float32)
-        R32 = torch.empty_like(U32)
+        U32 = state_in(args, case, dev).to(torch.float32)
+        R32 = torch.zeros_like(U32)
         if args.launch:
             s32.set_launch(*(int(v) for v in args.launch.split(",")))
         else:
-            s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=4, stream=stream)
-        ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, _lib.LAYOUT_LINES, stream),
+            s32.autotune(U32, R32, head_layout(args), reps=4, stream=stream)
+        ms32 = time_launches(lambda: s32.compute_rhs_dev(U32, R32, head_layout(args), stream),
                              args.steps, args.warmup, stream)
         print(json.dumps({"kernel": f"volume_kernel<4, float, float, EPI_RHS, "
                                     f"{s32.get_launch()[2]}>",
@@ -855,8 +892,9 @@ def main():
 
     s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U = torch.from_numpy(synth.to_lines(case.U, E, 4 + 1)).to(dev)
-    R = torch.empty_like(U)
+    lay = head_layout(args)
+    U = state_in(args, case, dev)
+    R = torch.zeros_like(U)
     # launch shape from the library's autotuner (workgroups per CU, XCD order,
     # grid lines per workgroup)
     if args.launch:
@@ -865,14 +903,14 @@ def main():
         log(f"launch shape fixed: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, "
             f"{tune[2]} lines per workgroup")
     else:
-        tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=4, stream=stream)
+        tune = s64.autotune(U, R, lay, reps=4, stream=stream)
         log(f"autotune: {tune[0]} workgroups/CU, xcd_remap={tune[1]}, {tune[2]} lines per "
             f"workgroup, {tune[3]:.1f} us")
     # One input/output pair: each launch reads 1.44 GB and writes 1.31 GB, so
     # the 256 MB last-level cache holds under a tenth of it between launches
     # and the state is read from HBM (at 64x64x8 the timed launches had to
     # cycle over several sets for that).  HIP events on the launch stream.
-    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
+    ms = time_launches(lambda: s64.compute_rhs_dev(U, R, lay, stream),
                        args.steps, args.warmup, stream)
     gdof = dof * 5 / (ms * 1e-3) / 1e9
     bytes_per = s64.rhs_bytes()
@@ -883,7 +921,7 @@ def main():
                                 "wg_lines": tune[2],
                                 "us": round(tune[3], 2) if tune[3] == tune[3] else None}}
     if not args.rhs_only:
-        r64 = R.cpu().numpy()
+        r64 = per_field(R.cpu().numpy(), args, case)
         extras.update(measure_mixed_and_steps(args, case, s64, U, r64, stream, label))
         extras["mixed_vs_fp64_gdof_ratio"] = round(extras["mixed_gdof_s"] / gdof, 3)
         del r64
@@ -908,9 +946,11 @@ def main():
             f"({cpu['cpu_model']})")
 
     # HBM bytes and FP64 FLOPs per launch of this kernel, from rocprofv3
-    # counter runs of the same command (profiles/volume_rhs_fp64_128x128x16_n4.md)
+    # counter runs of the same command (profiles/volume_rhs_fp64_128x128x16_n4.md,
+    # and volume_rhs_fp64_lines_128x128x16_n4.md for the line layout)
     traffic, tsrc, flops = None, None, None
-    tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_128x128x16_n4.traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "volume_rhs_fp64_%s128x128x16_n4.traffic.json"
+                         % ("" if args.layout == "blocks" else "lines_"))
     if os.path.exists(tfile):
         tj = json.load(open(tfile))
         traffic = tj.get("bytes_per_launch")
@@ -934,7 +974,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "DG RHS, uniform 128x128x16 octree, N=4, tanh shelf, FP64, "
-                        "structure-of-arrays (xi-planes) in HBM",
+                        + ("xi-planes in blocks of 20 elements (DRF_LAYOUT_BLOCKS)"
+                           if args.layout == "blocks" else
+                           "whole-mesh xi-planes (DRF_LAYOUT_LINES)") + " in HBM",
+            "layout": args.layout,
             "elements": E,
             "order": 4,
             "dof_per_field": dof,

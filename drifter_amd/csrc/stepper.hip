@@ -30,10 +30,13 @@ __global__ void __launch_bounds__(256)
         const int64_t e = b * epb + le;
         if (le < epb && e < E) {
             double hm = -INFINITY, um = 0.0, vm = 0.0;
+            // the line's offsets, once per line (the element part of the
+            // blocked layout takes a division)
+            const int64_t uo = node_offset(U, e, L, 0), ho = node_offset(h, e, L, 0);
             for (int i = 0; i < n; ++i) {
-                const int64_t o = node_offset(U, e, L, i);
+                const int64_t o = uo + (int64_t)i * U.is;
                 const double eta = U.p[2 * U.fs + o];
-                const double H = eta + h.p[node_offset(h, e, L, i)];
+                const double H = eta + h.p[ho + (int64_t)i * h.is];
                 const double hi = (H > 1e-10) ? 1.0 / H : 0.0;
                 const double u = U.p[0 * U.fs + o] * hi;
                 const double v = U.p[1 * U.fs + o] * hi;
