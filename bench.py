@@ -678,20 +678,25 @@ def measure_adapt_cycle(stream):
     src, boxes = amr.refine_transfer_map(old_b, new_b)
     src_d, boxes_d = torch.from_numpy(src).to("cuda"), torch.from_numpy(boxes).to("cuda")
     t["transfer_map_host"] = time.perf_counter() - t0
-    def transfer():
-        return [amr.interpolate_boxes(order, U[:, f * nd:(f + 1) * nd], src_d, boxes_d, stream)
-                for f in range(5)] + [amr.interpolate_boxes(order, h, src_d, boxes_d, stream)]
+    cols = [U[:, f * nd:(f + 1) * nd] for f in range(5)] + [h]
 
-    transfer()  # the first call's output allocations are not the transfer's cost
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fields = transfer()
-    torch.cuda.synchronize()
-    t["transfer_device"] = time.perf_counter() - t0
+    def transfer():  # the five state fields and the bathymetry in one launch
+        return amr.interpolate_boxes_fields(order, cols, src_d, boxes_d, stream)
+
+    def transfer_by_field():
+        return [amr.interpolate_boxes(order, c, src_d, boxes_d, stream) for c in cols]
+
+    for fn, key in ((transfer, "transfer_device"), (transfer_by_field, "transfer_by_field_device")):
+        fn()  # the first call's output allocations are not the transfer's cost
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fields = fn()
+        torch.cuda.synchronize()
+        t[key] = time.perf_counter() - t0
+        del fields
     out = {"adapt_cycle_elements_before_after": [int(E), int(new_b.shape[0])],
            "adapt_cycle_refined": int(els.size)}
     out.update({f"adapt_cycle_{k}_ms": round(v * 1e3, 3) for k, v in t.items()})
-    del fields
     log(f"adapt cycle: {out}")
     return out
 
@@ -856,6 +861,9 @@ def main():
     ap.add_argument("--profile-faces", action="store_true",
                     help="launch the face term only, FP64, on the headline mesh with every "
                          "face conforming (for its counters)")
+    ap.add_argument("--profile-adapt", action="store_true",
+                    help="run the adaptation-cycle leg only (estimator, refinement, "
+                         "connectivity, transfer map and the six-field transfer), three times")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time plain copies of known size instead (HBM counter calibration "
                          "and copy ceiling)")
@@ -868,6 +876,13 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.steps, args.warmup)
+        return
+
+    if args.profile_adapt:
+        import torch
+
+        for _ in range(3):
+            measure_adapt_cycle(torch.cuda.current_stream())
         return
 
     import torch

@@ -106,6 +106,7 @@ SIGNATURES = {
     "drf_project_from_children_dev": (_I, [_I, _I, _L, _P, _L, _P, _L, _P]),
     "drf_transfer_solution_dev": (_I, [_I, _L, _P, _L, _L, _P, _P, _L, _P]),
     "drf_interpolate_boxes_dev": (_I, [_I, _L, _L, _P, _P, _P, _L, _P, _L, _P]),
+    "drf_interpolate_boxes_fields_dev": (_I, [_I, _L, _L, _P, _P, _I, _P, _P, _P, _P, _P]),
     "drf_transfer_solution": (_I, [_I, _L, _PD, _L, _P, _PD, _I]),
     "drf_refine_transfer_map": (_I, [_L, _PD, _L, _PD, _P, _PD]),
     "drf_modal_matrices": (_I, [_I, _PD, _PD]),

@@ -11,6 +11,8 @@ on the HIP library through the C ABI.
 Fields are one nodal value set per element, device float64 tensors of shape
 (E, (N+1)^3) (or any element stride); errors raise DrfError.
 """
+import ctypes as C
+
 import numpy as np
 
 from . import _lib
@@ -112,6 +114,34 @@ def interpolate_boxes(order, U, src, boxes, stream=None):
                                                 _ptr(U), _stride(U, nd), _ptr(out), nd,
                                                 _sptr(stream)))
     return out
+
+
+def interpolate_boxes_fields(order, fields, src, boxes, stream=None):
+    """interpolate_boxes for up to 8 fields of one mesh in one launch: fields
+    is a list of (E, >= nd) device float64 views (rows may be strided, as the
+    columns of a packed state are); returns one (K, nd) output per field,
+    each bit-identical to interpolate_boxes of that field."""
+    import torch
+
+    nf = len(fields)
+    if not 1 <= nf <= 8:
+        raise ValueError("1 to 8 fields per call")
+    nd = (order + 1) ** 3
+    K = src.numel()
+    n_elem = fields[0].shape[0]
+    for U in fields:
+        if U.shape[0] != n_elem or U.dtype != torch.float64 or not U.is_cuda:
+            raise ValueError("fields must be float64 device tensors with one row per element")
+        if U.dim() == 2 and (U.shape[1] < nd or U.stride(1) != 1):
+            raise ValueError("each element's nd values must be contiguous")
+    outs = [torch.empty((K, nd), dtype=torch.float64, device=fields[0].device) for _ in fields]
+    Up = (C.c_void_p * nf)(*[_ptr(U) for U in fields])
+    es = (C.c_int64 * nf)(*[_stride(U, nd) for U in fields])
+    Op = (C.c_void_p * nf)(*[_ptr(o) for o in outs])
+    os_ = (C.c_int64 * nf)(*([nd] * nf))
+    check(_lib.load().drf_interpolate_boxes_fields_dev(order, K, n_elem, _ptr(src), _ptr(boxes),
+                                                       nf, Up, es, Op, os_, _sptr(stream)))
+    return outs
 
 
 def transfer_to_refined_mesh(order, U, old_bounds, new_bounds, stream=None):

@@ -35,13 +35,35 @@ struct FromChildrenOps {
     int8_t half[3][kMaxN];
 };
 
+// The fields one launch transfers: field f reads element e at U[f] + e es[f]
+// and writes output element o at out[f] + o os[f].  Boxes mode takes up to
+// kMaxBoxFields fields; Transfer and Children take one.
+constexpr int kMaxBoxFields = 8;
+struct BoxFields {
+    const double *U[kMaxBoxFields];
+    int64_t es[kMaxBoxFields];
+    double *out[kMaxBoxFields];
+    int64_t os[kMaxBoxFields];
+    int nf;
+};
+inline BoxFields one_field(const double *U, int64_t es, double *out, int64_t os)
+{
+    BoxFields bf{};
+    bf.U[0] = U;
+    bf.es[0] = es;
+    bf.out[0] = out;
+    bf.os[0] = os;
+    bf.nf = 1;
+    return bf;
+}
+
 // Transfer: n_out = n_keep + 8 n_parent, parents idx[n_parent]; Children:
 // every one of the n_src elements is a parent, n_out = n_src nchild(mask);
 // Boxes: idx[n_out] sources and boxes[6 n_out].
 hipError_t launch_interp_boxes(int n, BoxMode mode, int mask, int64_t n_out, int64_t n_src,
-                               const double *U, int64_t es, int64_t n_keep, int64_t n_parent,
+                               const BoxFields &bf, int64_t n_keep, int64_t n_parent,
                                const int64_t *idx, const double *boxes, const NodeSet &nodes,
-                               double *out, int64_t os, hipStream_t s);
+                               hipStream_t s);
 hipError_t launch_from_children(int n, int mask, int64_t n_parent, const double *children,
                                 int64_t cs, const FromChildrenOps &ops, double *U, int64_t es,
                                 hipStream_t s);

@@ -99,10 +99,10 @@ drf_status drf_project_to_children_dev(int order, int mask, int64_t n_parent, co
     DRF_REQUIRE(U && children, DRF_ERR_ARG, "drf_project_to_children_dev: null pointer");
     NodeSet ns;
     DRF_REQUIRE(node_set(n, ns), DRF_ERR_ARG, "drf_project_to_children_dev: LGL nodes failed");
-    DRF_HIP_CHECK(launch_interp_boxes(n, BoxMode::Children, mask, n_parent * num_children(mask),
-                                      n_parent, (const double *)U, elem_stride, 0, 0, nullptr,
-                                      nullptr, ns, (double *)children, child_stride,
-                                      (hipStream_t)stream));
+    DRF_HIP_CHECK(launch_interp_boxes(
+        n, BoxMode::Children, mask, n_parent * num_children(mask), n_parent,
+        one_field((const double *)U, elem_stride, (double *)children, child_stride), 0, 0, nullptr,
+        nullptr, ns, (hipStream_t)stream));
     return DRF_OK;
 }
 
@@ -145,10 +145,10 @@ drf_status drf_transfer_solution_dev(int order, int64_t n_elem, const void *U, i
                 "drf_transfer_solution_dev: null pointer");
     NodeSet ns;
     DRF_REQUIRE(node_set(n, ns), DRF_ERR_ARG, "drf_transfer_solution_dev: LGL nodes failed");
-    DRF_HIP_CHECK(launch_interp_boxes(n, BoxMode::Transfer, 7, n_out, n_elem, (const double *)U,
-                                      elem_stride, n_keep, n_refine, (const int64_t *)parents,
-                                      nullptr, ns, (double *)U_new, new_stride,
-                                      (hipStream_t)stream));
+    DRF_HIP_CHECK(launch_interp_boxes(
+        n, BoxMode::Transfer, 7, n_out, n_elem,
+        one_field((const double *)U, elem_stride, (double *)U_new, new_stride), n_keep, n_refine,
+        (const int64_t *)parents, nullptr, ns, (hipStream_t)stream));
     return DRF_OK;
 }
 
@@ -189,26 +189,50 @@ drf_status drf_transfer_solution(int order, int64_t n_elem, const double *U, int
     return DRF_OK;
 }
 
+drf_status drf_interpolate_boxes_fields_dev(int order, int64_t n_out, int64_t n_elem,
+                                            const void *src, const void *boxes, int n_fields,
+                                            const void *const *U, const int64_t *elem_stride,
+                                            void *const *U_new, const int64_t *new_stride,
+                                            void *stream)
+{
+    DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_UNSUPPORTED,
+                "drf_interpolate_boxes_dev: order out of range");
+    DRF_REQUIRE(n_fields >= 1 && n_fields <= kMaxBoxFields, DRF_ERR_ARG,
+                "drf_interpolate_boxes_fields_dev: 1 to " + std::to_string(kMaxBoxFields) +
+                    " fields per call");
+    DRF_REQUIRE(U && elem_stride && U_new && new_stride, DRF_ERR_ARG,
+                "drf_interpolate_boxes_fields_dev: null field array");
+    const int n = order + 1, nd = n * n * n;
+    DRF_REQUIRE(n_out >= 0 && n_elem >= 0, DRF_ERR_ARG, "drf_interpolate_boxes_dev: bad sizes");
+    BoxFields bf{};
+    bf.nf = n_fields;
+    for (int f = 0; f < n_fields; ++f) {
+        DRF_REQUIRE(elem_stride[f] >= nd && new_stride[f] >= nd, DRF_ERR_ARG,
+                    "drf_interpolate_boxes_dev: bad sizes (strides must be >= (N+1)^3)");
+        DRF_REQUIRE(n_out == 0 || (U_new[f] && (n_elem == 0 || U[f])), DRF_ERR_ARG,
+                    "drf_interpolate_boxes_dev: null pointer");
+        bf.U[f] = (const double *)U[f];
+        bf.es[f] = elem_stride[f];
+        bf.out[f] = (double *)U_new[f];
+        bf.os[f] = new_stride[f];
+    }
+    if (n_out == 0)
+        return DRF_OK;
+    DRF_REQUIRE(src && boxes, DRF_ERR_ARG, "drf_interpolate_boxes_dev: null pointer");
+    NodeSet ns;
+    DRF_REQUIRE(node_set(n, ns), DRF_ERR_ARG, "drf_interpolate_boxes_dev: LGL nodes failed");
+    DRF_HIP_CHECK(launch_interp_boxes(n, BoxMode::Boxes, 7, n_out, n_elem, bf, 0, 0,
+                                      (const int64_t *)src, (const double *)boxes, ns,
+                                      (hipStream_t)stream));
+    return DRF_OK;
+}
+
 drf_status drf_interpolate_boxes_dev(int order, int64_t n_out, int64_t n_elem, const void *src,
                                      const void *boxes, const void *U, int64_t elem_stride,
                                      void *U_new, int64_t new_stride, void *stream)
 {
-    DRF_REQUIRE(order >= 1 && order + 1 <= kMaxN, DRF_ERR_UNSUPPORTED,
-                "drf_interpolate_boxes_dev: order out of range");
-    const int n = order + 1, nd = n * n * n;
-    DRF_REQUIRE(n_out >= 0 && n_elem >= 0 && elem_stride >= nd && new_stride >= nd, DRF_ERR_ARG,
-                "drf_interpolate_boxes_dev: bad sizes");
-    if (n_out == 0)
-        return DRF_OK;
-    DRF_REQUIRE(src && boxes && U_new && (n_elem == 0 || U), DRF_ERR_ARG,
-                "drf_interpolate_boxes_dev: null pointer");
-    NodeSet ns;
-    DRF_REQUIRE(node_set(n, ns), DRF_ERR_ARG, "drf_interpolate_boxes_dev: LGL nodes failed");
-    DRF_HIP_CHECK(launch_interp_boxes(n, BoxMode::Boxes, 7, n_out, n_elem, (const double *)U,
-                                      elem_stride, 0, 0, (const int64_t *)src,
-                                      (const double *)boxes, ns, (double *)U_new, new_stride,
-                                      (hipStream_t)stream));
-    return DRF_OK;
+    return drf_interpolate_boxes_fields_dev(order, n_out, n_elem, src, boxes, 1, &U, &elem_stride,
+                                            &U_new, &new_stride, stream);
 }
 
 drf_status drf_refine_transfer_map(int64_t n_old, const double *old_bounds, int64_t n_new,

@@ -566,6 +566,18 @@ drf_status drf_transfer_solution(int order, int64_t n_elem, const double *U, int
 drf_status drf_interpolate_boxes_dev(int order, int64_t n_out, int64_t n_elem, const void *src,
                                      const void *boxes, const void *U, int64_t elem_stride,
                                      void *U_new, int64_t new_stride, void *stream);
+/* The same for n_fields (1 to 8) fields of one mesh in one launch (no
+ * counterpart: the reference transfers field by field): field f reads
+ * element e at U[f] + e elem_stride[f] and writes output element k at
+ * U_new[f] + k new_stride[f] (host arrays of device pointers and strides).
+ * Each field's values are those drf_interpolate_boxes_dev gives, bit for
+ * bit; the sources, boxes and interval matrices are set up once per batch
+ * of outputs for all fields. */
+drf_status drf_interpolate_boxes_fields_dev(int order, int64_t n_out, int64_t n_elem,
+                                            const void *src, const void *boxes, int n_fields,
+                                            const void *const *U, const int64_t *elem_stride,
+                                            void *const *U_new, const int64_t *new_stride,
+                                            void *stream);
 /* Host: src and boxes of drf_interpolate_boxes_dev from the leaf bounds
  * (E*6, as drf_mesh_elements writes them) before and after
  * drf_mesh_refine (refine + balance, octree_adapter.cpp:184-203).  Leaves

@@ -274,6 +274,44 @@ def test_interpolate_boxes_any_boxes_invalid_sources_grid_stride(torch, orc, ord
     np.testing.assert_array_equal(got[whole], U[src[whole]])
 
 
+@pytest.mark.parametrize("order", [1, 4, 8])
+def test_interpolate_boxes_fields_bit_identical_to_one_field(torch, order):
+    """Five columns of a packed state (element stride 5 nd) and one contiguous
+    field in one launch give, field by field, the bits of one launch per
+    field: copies, both halves, arbitrary intervals, invalid sources, and more
+    outputs than one pass of the grid."""
+    from drifter_amd import amr
+
+    n = order + 1
+    nd = n ** 3
+    epb = max(1024 // nd, 1)
+    n_src, n_out = 257, 4096 * epb + 131
+    rng = np.random.default_rng(100 + order)
+    P = torch.from_numpy(rng.standard_normal((n_src, 5 * nd))).cuda()
+    h = torch.from_numpy(rng.standard_normal((n_src, nd))).cuda()
+    src = rng.integers(0, n_src, n_out)
+    src[[3, 11]] = -1
+    src[n_out - 1] = n_src
+    boxes = np.empty((n_out, 6))
+    for a in range(3):
+        kind = rng.integers(0, 4, n_out)
+        r = np.sort(rng.uniform(-1, 1, (n_out, 2)), axis=1)
+        boxes[:, 2 * a] = np.where(kind == 2, 0.0, np.where(kind == 3, r[:, 0], -1.0))
+        boxes[:, 2 * a + 1] = np.where(kind == 1, 0.0, np.where(kind == 3, r[:, 1], 1.0))
+    src_d, boxes_d = torch.from_numpy(src).cuda(), torch.from_numpy(boxes).cuda()
+    fields = [P[:, f * nd:(f + 1) * nd] for f in range(5)] + [h]
+    print(f"[interpolate_boxes_fields] order={order} n_out={n_out}", flush=True)
+    many = amr.interpolate_boxes_fields(order, fields, src_d, boxes_d)
+    for f, U in enumerate(fields):
+        one = amr.interpolate_boxes(order, U, src_d, boxes_d)
+        torch.cuda.synchronize()
+        a, b = many[f].cpu().numpy(), one.cpu().numpy()
+        assert np.array_equal(a, b, equal_nan=True), f
+        assert np.isnan(a[src < 0]).all() and not np.isnan(a[src == 0]).any()
+    with pytest.raises(ValueError):
+        amr.interpolate_boxes_fields(order, fields + fields[:3], src_d, boxes_d)
+
+
 def test_transfer_many_copies_few_parents(torch, orc):
     """transfer_solution's copies and children share one launch: more kept
     elements than one pass of the copy workgroups (4,096 x 16) beside a few
