@@ -37,7 +37,7 @@ struct LineMap {
     int P, epw, lpl; // group size, elements per wave, lines per lane
 };
 
-__host__ __device__ inline LineMap line_map(int n)
+__host__ __device__ constexpr LineMap line_map(int n)
 {
     const int n2 = n * n;
     int P = 1;
@@ -66,6 +66,37 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A wave's elements e0 .. e0 + epw - 1 (cnt of them), nodal values in
+// registers: lane l holds values l, l + 64, ... of the wave's slice.  The
+// kernels load the next elements this way while they work on the current
+// ones, then store them to the slice.
+template <int n> struct Slice {
+    static constexpr int nd = n * n * n;
+    static constexpr int VPL = (line_map(n).epw * nd + 63) / 64;
+    double v[VPL];
+    __device__ __forceinline__ void load(const double *__restrict__ f, int64_t es, int64_t e0,
+                                         int cnt, int lane)
+    {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            const int idx = lane + 64 * k;
+            if (idx < cnt * nd) {
+                const int el = idx / nd, d = idx - el * nd;
+                v[k] = f[(e0 + el) * es + d];
+            }
+        }
+    }
+    __device__ __forceinline__ void store(double *u, int cnt, int lane) const
+    {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            const int idx = lane + 64 * k;
+            if (idx < cnt * nd)
+                u[idx] = v[k];
+        }
+    }
+};
+
 // Pass 1: raw indicator value[e] = (max - min) * h_char[e] and the
 // directional mask; partial[block] = max of the block's values.
 template <int n>
@@ -83,12 +114,17 @@ __global__ void __launch_bounds__(64 * kWaves)
     double *u = sm + (size_t)w * lm.epw * nd;
     double bmax = 0.0;
     const int64_t waves = (int64_t)gridDim.x * kWaves;
-    for (int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw; e0 < E; e0 += waves * lm.epw) {
-        const int cnt = (E - e0 < lm.epw) ? (int)(E - e0) : lm.epw;
-        for (int idx = lane; idx < cnt * nd; idx += 64) {
-            const int el = idx / nd, d = idx - el * nd;
-            u[idx] = field[(e0 + el) * es + d];
-        }
+    const int64_t step = waves * lm.epw;
+    auto count = [&](int64_t e) { return (E - e < lm.epw) ? (int)(E - e) : lm.epw; };
+    Slice<n> nx;
+    int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw;
+    if (e0 < E)
+        nx.load(field, es, e0, count(e0), lane);
+    for (; e0 < E; e0 += step) {
+        const int cnt = count(e0);
+        nx.store(u, cnt, lane);
+        if (e0 + step < E)
+            nx.load(field, es, e0 + step, count(e0 + step), lane);
         wave_sync_lds();
         double hi = -INFINITY, lo = INFINITY, vx = 0.0, vy = 0.0, vz = 0.0;
         if (g < cnt) {
@@ -240,12 +276,17 @@ __global__ void __launch_bounds__(64 * kWaves)
     const int g = lane / lm.P, gl = lane % lm.P;
     double *a = sm + (size_t)w * 2 * lm.epw * nd, *bb = a + lm.epw * nd;
     const int64_t waves = (int64_t)gridDim.x * kWaves;
-    for (int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw; e0 < E; e0 += waves * lm.epw) {
-        const int cnt = (E - e0 < lm.epw) ? (int)(E - e0) : lm.epw;
-        for (int idx = lane; idx < cnt * nd; idx += 64) {
-            const int el = idx / nd, d = idx - el * nd;
-            a[idx] = u[(e0 + el) * es + d];
-        }
+    const int64_t step = waves * lm.epw;
+    auto count = [&](int64_t e) { return (E - e < lm.epw) ? (int)(E - e) : lm.epw; };
+    Slice<n> nx;
+    int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw;
+    if (e0 < E)
+        nx.load(u, es, e0, count(e0), lane);
+    for (; e0 < E; e0 += step) {
+        const int cnt = count(e0);
+        nx.store(a, cnt, lane);
+        if (e0 + step < E)
+            nx.load(u, es, e0 + step, count(e0 + step), lane);
         wave_sync_lds();
         const bool act = g < cnt;
         double *ae = a + g * nd, *be = bb + g * nd;

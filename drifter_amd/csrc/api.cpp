@@ -199,8 +199,6 @@ drf_status volume(drf_ctx *c, const void *U, int layout, int mode, int acc_mode,
 
 drf_status rhs_into(drf_ctx *c, const void *U, void *R, int layout, hipStream_t s)
 {
-    DRF_REQUIRE(!(c->faces && layout == DRF_LAYOUT_BLOCKS), DRF_ERR_UNSUPPORTED,
-                "a face plan is attached: the face kernels take the PACKED, SOA or LINES layout");
     // An attached face plan's conforming interfaces go into the volume
     // kernel's last phase where the plan allows it; its mortars follow.
     VolFaces<double> f64{};

@@ -108,15 +108,20 @@ inline void layout_strides(int layout, int64_t E, int n, int nfields, int64_t &f
     }
 }
 
-// the element's part of an offset (every layout)
+// the element's part of an offset (every layout): element stride es, and in
+// the blocked layout block stride bs and bb elements per block
+__host__ __device__ inline int64_t elem_offset(int64_t e, int64_t es, int64_t bs, int64_t bb)
+{
+    if (bb) {
+        // 32-bit division where the element index allows it (every mesh here)
+        const int64_t blk = (e >> 31) ? e / bb : (int64_t)((uint32_t)e / (uint32_t)bb);
+        return blk * bs + (e - blk * bb) * es;
+    }
+    return e * es;
+}
 template <class T> __host__ __device__ inline int64_t elem_offset(const View<T> &v, int64_t e)
 {
-    if (v.bb) {
-        // 32-bit division where the element index allows it (every mesh here)
-        const int64_t blk = (e >> 31) ? e / v.bb : (int64_t)((uint32_t)e / (uint32_t)v.bb);
-        return blk * v.bs + (e - blk * v.bb) * v.es;
-    }
-    return e * v.es;
+    return elem_offset(e, v.es, v.bs, v.bb);
 }
 
 template <class T> __host__ __device__ inline int64_t node_offset(const View<T> &v, int64_t e, int L,

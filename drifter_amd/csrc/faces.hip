@@ -31,7 +31,7 @@ template <class T> __device__ __forceinline__ int64_t node_off(const FaceArgs<T>
                                                                int64_t e, int d)
 {
     const int i = d % n, L = d / n;
-    return e * a.es + (int64_t)L * a.ls + (int64_t)i * a.is;
+    return elem_offset(e, a.es, a.bs, a.bb) + (int64_t)L * a.ls + (int64_t)i * a.is;
 }
 
 // 1 / (w_i w_j w_k) of volume node d, as build_mass_matrices forms it
@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(256)
     const T *__restrict__ Uf = a.U + fld * a.fs;
     T *__restrict__ dUf = a.dU + fld * a.fs;
     auto line_off = [&](int64_t el, int line) {
-        return (uint32_t)((el * a.es + (int64_t)line * a.ls) * (int64_t)sizeof(T));
+        return (uint32_t)((elem_offset(el, a.es, a.bs, a.bb) + (int64_t)line * a.ls) *
+                          (int64_t)sizeof(T));
     };
     auto entry = [&](int64_t v, int line) {
         FaceNb r;

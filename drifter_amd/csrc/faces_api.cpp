@@ -135,8 +135,7 @@ template <class T> void fill_args(const drf_face_plan *p, FaceArgs<T> &a, const 
     std::memset(&a, 0, sizeof(a));
     a.U = (const T *)U;
     a.dU = (T *)dU;
-    int64_t bs = 0, bb = 0; // the face entry points refuse the blocked layout
-    layout_strides(layout, p->E, p->n, 5, a.fs, a.es, a.ls, a.is, bs, bb);
+    layout_strides(layout, p->E, p->n, 5, a.fs, a.es, a.ls, a.is, a.bs, a.bb);
     double x[kMaxN], w[kMaxN];
     lgl_nodes(p->n, x, w);
     for (int i = 0; i < p->n; ++i)
@@ -155,7 +154,8 @@ template <class T> void fill_args(const drf_face_plan *p, FaceArgs<T> &a, const 
 // state through 32-bit per-lane byte offsets: the element span must fit.
 template <class T> bool element_offsets_fit(const drf_face_plan *p, const FaceArgs<T> &a)
 {
-    const int64_t span = ((p->E - 1) * a.es + (int64_t)(p->nh - 1) * a.ls) * (int64_t)sizeof(T);
+    const int64_t span =
+        (elem_offset(p->E - 1, a.es, a.bs, a.bb) + (int64_t)(p->nh - 1) * a.ls) * (int64_t)sizeof(T);
     return span < ((int64_t)1 << 32);
 }
 
@@ -232,7 +232,10 @@ drf_status face_term(const drf_face_plan *p, const void *U, void *dU, int layout
 // do not fit), and the mortar groups on their own.
 template <class T> bool volume_faces(const drf_face_plan *p, int layout, VolFaces<T> *out)
 {
-    if (p->conf_mode != DRF_FACES_IN_VOLUME || p->conf.empty() || !p->d_wq)
+    // the volume kernel's face phase addresses unblocked states: a blocked
+    // state takes the element-ordered launch after the volume kernel
+    if (p->conf_mode != DRF_FACES_IN_VOLUME || p->conf.empty() || !p->d_wq ||
+        layout == DRF_LAYOUT_BLOCKS)
         return false;
     FaceArgs<T> a;
     fill_args<T>(p, a, nullptr, nullptr, layout);
@@ -495,10 +498,8 @@ drf_status drf_face_plan_lists(const drf_face_plan *p, int64_t *conforming, int6
 drf_status drf_face_term_dev(drf_face_plan *p, const void *U, void *dU, int layout, void *stream)
 {
     DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_term_dev: null plan");
-    DRF_REQUIRE(layout != DRF_LAYOUT_BLOCKS, DRF_ERR_UNSUPPORTED,
-                "drf_face_term_dev: the face kernels take the PACKED, SOA or LINES layout");
     DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA ||
-                    layout == DRF_LAYOUT_LINES,
+                    layout == DRF_LAYOUT_LINES || layout == DRF_LAYOUT_BLOCKS,
                 DRF_ERR_ARG, "drf_face_term_dev: unknown layout");
     DRF_REQUIRE(U && dU, DRF_ERR_ARG, "drf_face_term_dev: null state");
     DRF_HIP_CHECK(hipSetDevice(p->device));
@@ -511,10 +512,8 @@ drf_status drf_face_group_fluxes_dev(drf_face_plan *p, int group, const void *U,
     DRF_REQUIRE(p, DRF_ERR_ARG, "drf_face_group_fluxes_dev: null plan");
     DRF_REQUIRE(group >= 0 && group < (int)p->groups.size(), DRF_ERR_RANGE,
                 "drf_face_group_fluxes_dev: group out of range");
-    DRF_REQUIRE(layout != DRF_LAYOUT_BLOCKS, DRF_ERR_UNSUPPORTED,
-                "drf_face_group_fluxes_dev: the face kernels take the PACKED, SOA or LINES layout");
     DRF_REQUIRE(layout == DRF_LAYOUT_PACKED || layout == DRF_LAYOUT_SOA ||
-                    layout == DRF_LAYOUT_LINES,
+                    layout == DRF_LAYOUT_LINES || layout == DRF_LAYOUT_BLOCKS,
                 DRF_ERR_ARG, "drf_face_group_fluxes_dev: unknown layout");
     DRF_REQUIRE(U && out_a && out_b, DRF_ERR_ARG, "drf_face_group_fluxes_dev: null pointer");
     DRF_HIP_CHECK(hipSetDevice(p->device));

@@ -83,6 +83,7 @@ template <class T> struct FaceArgs {
     const T *U;
     T *dU;
     int64_t fs, es, ls, is;
+    int64_t bs, bb; // blocked layout: block stride and elements per block (0: unblocked)
     T w[kMaxN];     // 1D LGL weights of the element basis
     T rw[kMaxFaceQ]; // 1D weights of the conforming face rule (nq1 points)
     int nq1;

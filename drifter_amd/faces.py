@@ -108,7 +108,8 @@ class FacePlan:
     def face_term(self, U, dU, layout=_lib.LAYOUT_LINES, stream=None):
         """dU += the face term of state U (device tensors of the context's precision)."""
         s = self.solver
-        check(self._l.drf_face_term_dev(self._h, s._dev_state(U), s._dev_state(dU, "dU"),
+        check(self._l.drf_face_term_dev(self._h, s._dev_state(U, "U", layout),
+                                        s._dev_state(dU, "dU", layout),
                                         layout, _sptr(stream)))
 
     def group_fluxes(self, group, U, layout=_lib.LAYOUT_LINES, stream=None):
@@ -126,7 +127,8 @@ class FacePlan:
         b = torch.empty((cnt, 5, nh) if kind == KIND_CONFORMING else (cnt, 5, nf, nh), dtype=dt,
                         device=U.device)
         check(self._l.drf_face_group_fluxes_dev(
-            self._h, group, s._dev_state(U), layout, _dev_array(a, a.numel(), dt, "out_a"),
+            self._h, group, s._dev_state(U, "U", layout), layout,
+            _dev_array(a, a.numel(), dt, "out_a"),
             _dev_array(b, b.numel(), dt, "out_b"), _sptr(stream)))
         return a, b
 

@@ -22,8 +22,7 @@
  *                      (drf_state_len) and its slots past element E - 1
  *                      are neither read nor written.  The volume kernel
  *                      addresses a whole state in 32-bit byte offsets here
- *                      (under 4 GiB per state, DRF_ERR_UNSUPPORTED above);
- *                      a face plan attached to the context refuses it.
+ *                      (under 4 GiB per state, DRF_ERR_UNSUPPORTED above).
  *   fields: 0 Hu, 1 Hv, 2 eta, 3 HT, 4 HS
  *   full state (mode splitting): [e][Hu|Hv|eta|HT|HS|H|u|v|T|S|omega|rho][d]
  *
@@ -451,10 +450,11 @@ void drf_face_plan_destroy(drf_face_plan *plan);
 /* How the conforming interfaces run (no counterpart in the reference: the
  * launch organisation; every mode gives the face term to rounding):
  * DRF_FACES_IN_VOLUME (the default) adds them in the volume kernel's last
- * phase when the plan is attached to a right-hand side, and element by
- * element in drf_face_term_dev; DRF_FACES_BY_ELEMENT uses the element-ordered
- * launch after the volume kernel; DRF_FACES_BY_INTERFACE one launch per normal
- * axis, interface by interface.  Mortars always run as their own launches. */
+ * phase when the plan is attached to a right-hand side (on a DRF_LAYOUT_BLOCKS
+ * state: the element-ordered launch after it), and element by element in
+ * drf_face_term_dev; DRF_FACES_BY_ELEMENT uses the element-ordered launch
+ * after the volume kernel; DRF_FACES_BY_INTERFACE one launch per normal axis,
+ * interface by interface.  Mortars always run as their own launches. */
 enum { DRF_FACES_BY_INTERFACE = 0, DRF_FACES_BY_ELEMENT = 1, DRF_FACES_IN_VOLUME = 2 };
 drf_status drf_face_plan_set_conforming_mode(drf_face_plan *plan, int mode);
 drf_status drf_face_plan_info(const drf_face_plan *plan, int *n_groups, int64_t *n_conforming,

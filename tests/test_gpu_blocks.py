@@ -2,10 +2,10 @@
 gives the same bits as the line layout in every path that takes a layout:
 the right-hand side at every order, precision and workgroup size, the fused
 and un-fused Runge-Kutta steps, the CFL wave speed (also through the
-entry point that allocates nothing), and the autotuner.  A mesh of
+entry point that allocates nothing), the autotuner, and the right-hand side
+with a face plan attached.  A mesh of
 13 x 7 x 5 = 455 elements leaves the last block partly filled at every order
-(blocks of 128, 56, 32, 20, 3, 2, 2 and 1 elements at orders 1 to 8).
-A face plan attached to the context refuses the blocked layout."""
+(blocks of 128, 56, 32, 20, 3, 2, 2 and 1 elements at orders 1 to 8)."""
 import numpy as np
 import pytest
 
@@ -108,7 +108,10 @@ def test_autotune_on_blocks_keeps_results():
     s.close()
 
 
-def test_face_plan_refuses_blocks():
+def test_rhs_with_face_plan_blocks_bit_identical_to_lines():
+    """With a face plan attached the blocked state takes the element-ordered
+    face launch after the volume kernel, the line-layout state the fused face
+    phase: the same arithmetic, the same bits."""
     import torch
 
     from drifter_amd.mesh import OctreeAdapter
@@ -118,15 +121,15 @@ def test_face_plan_refuses_blocks():
     m.build_uniform(4, 4, 2)
     bounds, _, _ = m.elements()
     case = synth.make_case_on(bounds, 3, L, "seamount")
-    s, Ul, Ub, _ = _pair(case)
-    plan = faces.FacePlan.from_mesh(s, m, two_to_one=False, kind=faces.UPWIND)
-    plan.attach()
-    R = torch.empty_like(Ub)
-    with pytest.raises(_lib.DrfError) as ei:
-        s.compute_rhs_dev(Ub, R, _lib.LAYOUT_BLOCKS)
-    assert ei.value.status == _lib.DRF_ERR_UNSUPPORTED
-    Rl = torch.empty_like(Ul)
-    s.compute_rhs_dev(Ul, Rl, _lib.LAYOUT_LINES)
-    plan.detach()
-    plan.close()
-    s.close()
+    for prec in (_lib.PREC_FP64, _lib.PREC_MIXED):
+        s, Ul, Ub, B = _pair(case, prec)
+        plan = faces.FacePlan.from_mesh(s, m, two_to_one=False, kind=faces.UPWIND)
+        plan.attach()
+        Rl, Rb = torch.empty_like(Ul), torch.full_like(Ub, float("nan"))
+        s.compute_rhs_dev(Ul, Rl, _lib.LAYOUT_LINES)
+        s.compute_rhs_dev(Ub, Rb, _lib.LAYOUT_BLOCKS)
+        torch.cuda.synchronize()
+        assert _same(Rl, Rb, case, B), prec
+        plan.detach()
+        plan.close()
+        s.close()

@@ -263,6 +263,50 @@ def test_face_term_conforming_orders_agree(coast, layout, kind):
     s.close()
 
 
+@pytest.mark.parametrize("prec", [_lib.PREC_FP64, _lib.PREC_MIXED])
+def test_face_kernels_on_blocked_state_bit_identical(coast, prec):
+    """The coastline plan's face term (element by element and interface by
+    interface), every group's face vectors and the right-hand side with the
+    plan attached, on the blocked state, give the bits of the line layout."""
+    import torch
+
+    case, mesh = coast
+    s = _solver(case, prec)
+    plan = faces.FacePlan.from_mesh(s, mesh, two_to_one=True, kind=faces.LF_ADV, params=PARAMS)
+    E, n = case.E, case.n
+    B = _lib.load().drf_layout_block_elements(case.order)
+    dt = torch.float64 if prec == _lib.PREC_FP64 else torch.float32
+    Ul = torch.from_numpy(synth.to_lines(case.U, E, n)).cuda().to(dt)
+    Ub = torch.from_numpy(synth.to_blocks(case.U, E, n, B)).cuda().to(dt)
+
+    def same(a, b):
+        return np.array_equal(synth.from_lines(a.double().cpu().numpy(), E, n),
+                              synth.from_blocks(b.double().cpu().numpy(), E, n, B))
+
+    for mode in (faces.IN_VOLUME, faces.BY_INTERFACE):
+        plan.set_conforming_mode(mode)
+        Rl, Rb = torch.zeros_like(Ul), torch.zeros_like(Ub)
+        plan.face_term(Ul, Rl, _lib.LAYOUT_LINES)
+        plan.face_term(Ub, Rb, _lib.LAYOUT_BLOCKS)
+        torch.cuda.synchronize()
+        assert same(Rl, Rb), mode
+    for g in range(len(plan.groups())):
+        al, bl = plan.group_fluxes(g, Ul, _lib.LAYOUT_LINES)
+        ab, bb = plan.group_fluxes(g, Ub, _lib.LAYOUT_BLOCKS)
+        torch.cuda.synchronize()
+        assert torch.equal(al, ab) and torch.equal(bl, bb), g
+    plan.set_conforming_mode(faces.IN_VOLUME)
+    plan.attach()
+    Rl, Rb = torch.empty_like(Ul), torch.full_like(Ub, float("nan"))
+    s.compute_rhs_dev(Ul, Rl, _lib.LAYOUT_LINES)
+    s.compute_rhs_dev(Ub, Rb, _lib.LAYOUT_BLOCKS)
+    torch.cuda.synchronize()
+    assert same(Rl, Rb)
+    plan.detach()
+    plan.close()
+    s.close()
+
+
 def _rhs_with_faces(orc, case, U, cf, mt, mtypes):
     f = np.full(case.h.size, 1e-4)
     r = orc.rhs(case.order, U, case.h, f)
