@@ -69,7 +69,10 @@ __device__ __forceinline__ void wave_sync_lds()
 // A wave's elements e0 .. e0 + epw - 1 (cnt of them), nodal values in
 // registers: lane l holds values l, l + 64, ... of the wave's slice.  The
 // kernels load the next elements this way while they work on the current
-// ones, then store them to the slice.
+// ones, then store them to the slice.  Per-element scalars (h_char, the
+// feature scale) are loaded with them: a load issued after the prefetch
+// would make the wave wait for the prefetch too (gfx9 counts vector-memory
+// loads in one in-order counter).
 template <int n> struct Slice {
     static constexpr int nd = n * n * n;
     static constexpr int VPL = (line_map(n).epw * nd + 63) / 64;
@@ -118,13 +121,19 @@ __global__ void __launch_bounds__(64 * kWaves)
     auto count = [&](int64_t e) { return (E - e < lm.epw) ? (int)(E - e) : lm.epw; };
     Slice<n> nx;
     int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw;
+    double hc_next = 0.0; // h_char of the lane's element in the next slice
+    auto load_next = [&](int64_t e) {
+        nx.load(field, es, e, count(e), lane);
+        hc_next = (g < count(e)) ? hchar[e + g] : 0.0;
+    };
     if (e0 < E)
-        nx.load(field, es, e0, count(e0), lane);
+        load_next(e0);
     for (; e0 < E; e0 += step) {
         const int cnt = count(e0);
         nx.store(u, cnt, lane);
+        const double hc = hc_next;
         if (e0 + step < E)
-            nx.load(field, es, e0 + step, count(e0 + step), lane);
+            load_next(e0 + step);
         wave_sync_lds();
         double hi = -INFINITY, lo = INFINITY, vx = 0.0, vy = 0.0, vz = 0.0;
         if (g < cnt) {
@@ -171,7 +180,7 @@ __global__ void __launch_bounds__(64 * kWaves)
         vz = group_max(vz, lm.P);
         if (g < cnt) {
             const int64_t e = e0 + g;
-            const double val = (hi - lo) * hchar[e];
+            const double val = (hi - lo) * hc;
             bmax = dmax(bmax, val);
             if (gl == 0) {
                 const double mv = dmax(dmax(vx, vy), vz);
@@ -280,13 +289,19 @@ __global__ void __launch_bounds__(64 * kWaves)
     auto count = [&](int64_t e) { return (E - e < lm.epw) ? (int)(E - e) : lm.epw; };
     Slice<n> nx;
     int64_t e0 = ((int64_t)blockIdx.x * kWaves + w) * lm.epw;
+    double es_next = 1.0; // elem_scale of the lane's element in the next slice
+    auto load_next = [&](int64_t e) {
+        nx.load(u, es, e, count(e), lane);
+        es_next = (elem_scale && g < count(e)) ? elem_scale[e + g] : 1.0;
+    };
     if (e0 < E)
-        nx.load(u, es, e0, count(e0), lane);
+        load_next(e0);
     for (; e0 < E; e0 += step) {
         const int cnt = count(e0);
         nx.store(a, cnt, lane);
+        const double sc = elem_scale ? scale * es_next : scale; // used after the next load
         if (e0 + step < E)
-            nx.load(u, es, e0 + step, count(e0 + step), lane);
+            load_next(e0 + step);
         wave_sync_lds();
         const bool act = g < cnt;
         double *ae = a + g * nd, *be = bb + g * nd;
@@ -313,7 +328,6 @@ __global__ void __launch_bounds__(64 * kWaves)
         if (act) { // zeta: coef[L + n2 r] = sum_k Vi[r][k] a[L + n2 k], L = (p, q)
             const int64_t e = e0 + g;
             OUT *dst = out + e * out_stride + out_offset;
-            const double sc = elem_scale ? scale * elem_scale[e] : scale;
             for (int L = gl; L < n2; L += lm.P)
                 for (int r = 0; r < n; ++r) {
                     double s = 0.0;
