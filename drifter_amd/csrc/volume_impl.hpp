@@ -535,8 +535,13 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
             // Next batch of this workgroup, in flight during the other phases.
             const int64_t bn = b + G;
             const int64_t en = elem(bn);
+            // issued at raised wave priority, so that the SIMD's other wave
+            // does not interleave its arithmetic with the 30 loads (1 % on
+            // the FP64 and mixed RHS at 128x128x16)
+            __builtin_amdgcn_s_setprio(2);
             if (in_tile && bn < nbatch && en < geo.E)
                 load_lines<n, EPB, TS, TC>(nxt, U, geo, bn, en, L);
+            __builtin_amdgcn_s_setprio(0);
         }
 
         // ---------------- P2a: eta-line (i, k) = (L % n, L / n) ----------------
