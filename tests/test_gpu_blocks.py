@@ -133,3 +133,37 @@ def test_rhs_with_face_plan_blocks_bit_identical_to_lines():
         plan.detach()
         plan.close()
         s.close()
+
+
+def test_adaptive_steps_and_graph_on_blocks():
+    """SimulationDriver::step on the device (CFL dt from the wave speed, fused
+    RK4) on the blocked state gives the line layout's state, dt and time;
+    a captured graph of those steps replays them exactly, with eager work
+    between the replays."""
+    import torch
+
+    case = synth.make_case(13, 7, 5, 3, 10000.0, "seamount")
+    s, Ul, Ub, B = _pair(case)
+    z = lambda: torch.zeros(1, dtype=torch.float64, device="cuda")  # noqa: E731
+    dtl, tl, dtb, tb = z(), z(), z(), z()
+    for _ in range(3):
+        s.rk_step_adaptive_dev(_lib.RK_RK4, Ul, dtl, tl, cfl=0.5, layout=_lib.LAYOUT_LINES)
+        s.rk_step_adaptive_dev(_lib.RK_RK4, Ub, dtb, tb, cfl=0.5, layout=_lib.LAYOUT_BLOCKS)
+    torch.cuda.synchronize()
+    assert _same(Ul, Ub, case, B)
+    assert dtl.item() == dtb.item() > 0 and tl.item() == tb.item()
+    Ug = Ub.clone()
+    dtg, tg = dtb.clone(), tb.clone()
+    g = s.capture_adaptive_steps(_lib.RK_RK4, Ug, dtg, tg, n_steps=2, cfl=0.5,
+                                 layout=_lib.LAYOUT_BLOCKS)
+    for rep in range(3):
+        for _ in range(2):
+            s.rk_step_adaptive_dev(_lib.RK_RK4, Ub, dtb, tb, cfl=0.5, layout=_lib.LAYOUT_BLOCKS)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.isfinite(torch.randn(1000, device="cuda").sum())
+        assert torch.equal(Ug, Ub), rep
+        assert tg.item() == tb.item() and dtg.item() == dtb.item(), rep
+    g.close()
+    s.close()
