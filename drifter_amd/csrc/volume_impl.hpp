@@ -98,8 +98,8 @@ __device__ __forceinline__ void sto(TS *ubase, uint32_t off, TS v)
 // host checks that every offset of the launch fits in 32 bits.
 //
 // A view without blocks: the bathymetry and Coriolis lines (always in the
-// line layout), and the fused face phase (face plans refuse the blocked
-// layout).
+// line layout, no larger than one field of the state), and the fused face
+// phase (which runs on unblocked states only).
 template <class T> __device__ __forceinline__ uint32_t flat_off(const View<T> &v, int64_t e, int L)
 {
     return ((uint32_t)e * (uint32_t)v.es + (uint32_t)L * (uint32_t)v.ls) * (uint32_t)sizeof(T);
@@ -536,8 +536,8 @@ __global__ void __launch_bounds__((Cfg<N, LW>::BLOCK)) __attribute__((amdgpu_wav
             const int64_t bn = b + G;
             const int64_t en = elem(bn);
             // issued at raised wave priority, so that the SIMD's other wave
-            // does not interleave its arithmetic with the 30 loads (1 % on
-            // the FP64 and mixed RHS at 128x128x16)
+            // does not interleave its arithmetic with these loads (6 (N + 1)
+            // per lane; 1 % on the FP64 and mixed RHS at 128x128x16, N = 4)
             __builtin_amdgcn_s_setprio(2);
             if (in_tile && bn < nbatch && en < geo.E)
                 load_lines<n, EPB, TS, TC>(nxt, U, geo, bn, en, L);
