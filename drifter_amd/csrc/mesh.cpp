@@ -252,10 +252,23 @@ bool needs_split(const drf_mesh &m, int64_t id)
 // whose check split a neighbour; and of these only the ones that a parallel
 // filter against the tree at the start of the pass flags.  The splits and
 // their order are those of rescanning every leaf in every pass.
-void balance(drf_mesh &m)
+//
+// first_new: nodes from this index on were created since the tree was last
+// balanced (0: check every leaf).  Splits only make a leaf's neighbours
+// finer, so a leaf of a balanced tree keeps needing none: the first pass
+// checks the new leaves only, and finds the splits (in leaf order) of a
+// full scan.
+void balance(drf_mesh &m, int64_t first_new = 0)
 {
     bool changed = true;
-    std::vector<int64_t> snapshot = m.leaves;
+    std::vector<int64_t> snapshot;
+    if (first_new > 0) {
+        for (int64_t id : m.leaves)
+            if (id >= first_new)
+                snapshot.push_back(id);
+    } else {
+        snapshot = m.leaves;
+    }
     while (changed) {
         changed = false;
         const int64_t first_new = (int64_t)m.nodes.size();
@@ -390,6 +403,9 @@ drf_status drf_mesh_refine(drf_mesh *m, int64_t count, const int64_t *elems, con
     if (count > 0)
         m->nodes.reserve(m->nodes.size() +
                          8 * std::min<size_t>((size_t)count, m->leaves.size()));
+    // every mesh this API hands out is 2:1 balanced (build_uniform, refine
+    // and balance leave it so): the balance below starts from the new leaves
+    const int64_t first_new = (int64_t)m->nodes.size();
     for (int64_t i = 0; i < count; ++i) {
         const int64_t e = elems[i];
         if (e < 0 || e >= (int64_t)m->leaves.size())
@@ -401,7 +417,7 @@ drf_status drf_mesh_refine(drf_mesh *m, int64_t count, const int64_t *elems, con
         create_children(*m, id);
     }
     number_leaves(*m);
-    balance(*m);
+    balance(*m, first_new);
     return DRF_OK;
 }
 
