@@ -52,7 +52,7 @@ def time_launches(fn, steps, warmup, stream):
     return t0.elapsed_time(t1) / steps  # ms per call
 
 
-def time_rotating(solver, U, R, nset, steps, warmup, stream):
+def time_rotating(solver, U, R, nset, steps, warmup, stream, layout=None):
     """ms per compute_rhs_dev launch cycling over nset input/output sets
     (U, R and nset - 1 copies), so that each launch reads its state from HBM
     rather than from the last-level cache."""
@@ -67,7 +67,8 @@ def time_rotating(solver, U, R, nset, steps, warmup, stream):
     def one():
         k = it[0] % nset
         it[0] += 1
-        solver.compute_rhs_dev(Us[k], Rs[k], _lib.LAYOUT_LINES, stream)
+        solver.compute_rhs_dev(Us[k], Rs[k], _lib.LAYOUT_LINES if layout is None else layout,
+                               stream)
 
     ms = time_launches(one, steps, warmup, stream)
     del Us, Rs
@@ -351,12 +352,13 @@ def measure_config2(args, stream):
     out = {}
     s64 = PrimitiveEquationsSolver(4, _lib.PREC_FP64)
     s64.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U = torch.from_numpy(synth.to_lines(case.U, E, 5)).to("cuda")
-    R = torch.empty_like(U)
-    tune = s64.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
-    ms_pair = time_launches(lambda: s64.compute_rhs_dev(U, R, _lib.LAYOUT_LINES, stream),
+    lay = head_layout(args)
+    Uh = state_in(args, case, "cuda")  # the headline's layout, for the RHS timings
+    R = torch.empty_like(Uh)
+    tune = s64.autotune(Uh, R, lay, reps=8, stream=stream)
+    ms_pair = time_launches(lambda: s64.compute_rhs_dev(Uh, R, lay, stream),
                             args.steps, args.warmup, stream)
-    ms = time_rotating(s64, U, R, 4, args.steps, args.warmup, stream)
+    ms = time_rotating(s64, Uh, R, 4, args.steps, args.warmup, stream, lay)
     out["c2_fp64_us"] = round(ms * 1e3, 2)
     out["c2_fp64_one_pair_us"] = round(ms_pair * 1e3, 2)
     out["c2_fp64_gdof_s"] = round(dof * 5 / (ms * 1e-3) / 1e9, 3)
@@ -365,16 +367,17 @@ def measure_config2(args, stream):
                                "wg_lines": tune[2], "us": round(tune[3], 2)}
     s32 = PrimitiveEquationsSolver(4, _lib.PREC_MIXED)
     s32.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-    U32 = U.to(torch.float32)
+    U32 = Uh.to(torch.float32)
     R32 = torch.empty_like(U32)
-    s32.autotune(U32, R32, _lib.LAYOUT_LINES, reps=8, stream=stream)
-    ms32 = time_rotating(s32, U32, R32, 8, args.steps, args.warmup, stream)
+    s32.autotune(U32, R32, lay, reps=8, stream=stream)
+    ms32 = time_rotating(s32, U32, R32, 8, args.steps, args.warmup, stream, lay)
     out["c2_mixed_us"] = round(ms32 * 1e3, 2)
     out["c2_mixed_gdof_s"] = round(dof * 5 / (ms32 * 1e-3) / 1e9, 3)
     log(f"64x64x8 N=4: FP64 RHS {ms * 1e3:.1f} us from HBM ({ms_pair * 1e3:.1f} us on one pair), "
         f"mixed {ms32 * 1e3:.1f} us")
     s32.close()
-    del U32, R32
+    del U32, R32, Uh
+    U = torch.from_numpy(synth.to_lines(case.U, E, 5)).to("cuda")  # the steps below: LINES
 
     # the driver's step with the CFL dt (SimulationDriver::step): host loop
     # (wave speed to the host, compute_dt, fused step) vs device-resident
