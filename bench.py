@@ -428,11 +428,11 @@ def measure_coastline(args, stream):
                            ("mixed", _lib.PREC_MIXED, torch.float32)):
         s = PrimitiveEquationsSolver(3, prec)
         s.initialize(E, case.h, coriolis=CoriolisParameter(1e-4, 0.0, 0.0))
-        U = torch.from_numpy(synth.to_lines(case.U, E, 4)).to("cuda").to(dt)
+        U = state_in(args, case, "cuda").to(dt)  # the headline's layout
         R = torch.empty_like(U)
-        s.autotune(U, R, _lib.LAYOUT_LINES, reps=8, stream=stream)
+        s.autotune(U, R, head_layout(args), reps=8, stream=stream)
         # 8 sets (0.9-1.7 GB): state from HBM every launch, as for the value
-        ms = time_rotating(s, U, R, 8, args.steps, args.warmup, stream)
+        ms = time_rotating(s, U, R, 8, args.steps, args.warmup, stream, head_layout(args))
         out[f"coastline_39280_n3_{name}_gdof_s"] = round(E * nd * 5 / (ms * 1e-3) / 1e9, 3)
         out[f"coastline_39280_n3_{name}_us"] = round(ms * 1e3, 2)
     log(f"coastline N=3 RHS: {out}")
