@@ -4,6 +4,9 @@ oracle (PrimitiveEquationsSolver::compute_rhs,
 src/physics/primitive_equations.cpp:428-442; RK4Classic,
 src/solver/time_stepper.cpp:86-119) on about 400 sampled elements.
 
+The bench runs this workload on the blocked state layout; its RHS is checked
+bit-identical to the line layout's over the whole mesh.
+
 Bars, per field (Hu, Hv, eta, HT, HS), relative L2 over the sample:
   FP64 right-hand side and one fused RK4 step    <= 1e-12
   mixed (FP32 volume) right-hand side            <= 2e-5
@@ -92,6 +95,28 @@ def test_config4_fp64_rhs_and_rk4_step(orc, c4):
     errs = rel_l2_fields(got, _oracle_steps(orc, Usub, hsub, dt, 1), ND)
     print(f"  FP64 fused RK4 step rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) <= TOL64, errs
+
+
+def test_config4_fp64_rhs_on_the_blocked_state_equals_the_line_layout(c4):
+    """The bench's configuration: the full-size RHS on the blocked state (the
+    line layout's values, pinned to the oracle above) bit for bit, compared
+    on the device."""
+    import torch
+
+    case, idx, U, s, dt = c4
+    E = case.E
+    B = _lib.load().drf_layout_block_elements(ORDER)
+    Ub = torch.from_numpy(synth.to_blocks(case.U, E, N, B)).cuda()
+    nb = Ub.numel() // (5 * ND * B)
+    Rb = torch.empty_like(Ub)
+    s.compute_rhs_dev(Ub, Rb, _lib.LAYOUT_BLOCKS)
+    del Ub
+    R = torch.empty_like(U)
+    s.compute_rhs_dev(U, R, _lib.LAYOUT_LINES)
+    # blocked [block][field][i][e % B][L] -> line layout [field][i][e][L]
+    Rl = Rb.view(nb, 5, N, B, N * N).permute(1, 2, 0, 3, 4).reshape(5, N, nb * B, N * N)
+    assert torch.equal(Rl[:, :, :E, :], R.view(5, N, E, N * N))
+    del Rb, Rl, R
 
 
 def test_config4_mixed_rhs_and_100_rk4_steps(orc, c4):
