@@ -167,3 +167,30 @@ def test_adaptive_steps_and_graph_on_blocks():
         assert tg.item() == tb.item() and dtg.item() == dtb.item(), rep
     g.close()
     s.close()
+
+
+@pytest.mark.parametrize("order", [1, 4, 8])
+def test_mesh_smaller_than_one_block(order):
+    """A mesh of 8 elements (the uniform mesh rounds 7 x 1 x 1 up to a power
+    of two): one partly filled block at orders 1 and 4, whose blocks hold 128
+    and 20 elements; eight full blocks of one element at order 8."""
+    import torch
+
+    case = synth.make_case(7, 1, 1, order, 10000.0, "seamount")
+    E = case.E
+    s, Ul, Ub, B = _pair(case)
+    Rl, Rb = torch.empty_like(Ul), torch.full_like(Ub, float("nan"))
+    s.compute_rhs_dev(Ul, Rl, _lib.LAYOUT_LINES)
+    s.compute_rhs_dev(Ub, Rb, _lib.LAYOUT_BLOCKS)
+    s.rk_step_dev(_lib.RK_RK4, 0.0, 1e-3, Ul, _lib.LAYOUT_LINES, fused=True)
+    s.rk_step_dev(_lib.RK_RK4, 0.0, 1e-3, Ub, _lib.LAYOUT_BLOCKS, fused=True)
+    torch.cuda.synchronize()
+    assert _same(Rl, Rb, case, B) and _same(Ul, Ub, case, B)
+    # the slots past the last element keep what the caller put there
+    if B > E:
+        n = case.n
+        assert torch.isnan(Rb.view(1, 5, n, B, n * n)[:, :, :, E:, :]).all()
+        assert not torch.isnan(Rb.view(1, 5, n, B, n * n)[:, :, :, :E, :]).any()
+    assert s.max_wave_speed_dev(Ul, _lib.LAYOUT_LINES) == \
+        s.max_wave_speed_dev(Ub, _lib.LAYOUT_BLOCKS)
+    s.close()
