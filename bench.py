@@ -638,7 +638,8 @@ def measure_adapt_cycle(stream):
     """One adaptation cycle of the 64x64x8 N = 4 octree mesh, stage by stage
     (wall clock, each stage synchronised): the gradient estimator and its
     marking on the device, the refine list on the host, octree refine +
-    balance and the face connectivity on the host, the transfer map on the
+    balance and the face connectivity on the host (the reference's, and the
+    2:1 lists the face plans are built from), the transfer map on the
     host, and the transfer of the five state fields and the bathymetry to the
     refined mesh on the device."""
     import numpy as np
@@ -676,6 +677,9 @@ def measure_adapt_cycle(stream):
     t0 = time.perf_counter()
     m.build_face_connections()
     t["face_connections_host"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.build_face_connections(two_to_one=True)  # what the face plans are built from
+    t["face_connections_2to1_host"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     new_b, _, _ = m.elements()
     src, boxes = amr.refine_transfer_map(old_b, new_b)

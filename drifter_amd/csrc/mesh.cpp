@@ -117,6 +117,39 @@ void number_leaves(drf_mesh &m)
     }
 }
 
+// Renumbers after splits: a split leaf's place in the depth-first order is
+// taken by its subtree's leaves, so the previous leaf list with each split
+// leaf expanded depth first is the new order (the same numbering as
+// number_leaves, without walking the unsplit part of the tree).
+void renumber_leaves(drf_mesh &m)
+{
+    std::vector<int64_t> old;
+    old.swap(m.leaves);
+    m.leaves.reserve(m.nodes.size());
+    std::vector<int64_t> stack;
+    for (int64_t id0 : old) {
+        if (!m.nodes[id0].nchild) {
+            m.nodes[id0].leaf = (int64_t)m.leaves.size();
+            m.leaves.push_back(id0);
+            continue;
+        }
+        stack.assign(1, id0);
+        while (!stack.empty()) {
+            const int64_t id = stack.back();
+            stack.pop_back();
+            Node &nd = m.nodes[id];
+            if (!nd.nchild) {
+                nd.leaf = (int64_t)m.leaves.size();
+                m.leaves.push_back(id);
+                continue;
+            }
+            nd.leaf = -1;
+            for (int c = nd.nchild - 1; c >= 0; --c)
+                stack.push_back(nd.child0 + c);
+        }
+    }
+}
+
 bool inside(const double *b, const double *p)
 {
     return p[0] >= b[0] && p[0] <= b[1] && p[1] >= b[2] && p[1] <= b[3] && p[2] >= b[4] &&
@@ -153,14 +186,19 @@ int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
     if (cur < 0)
         cur = 0;
     while (m.nodes[cur].nchild) {
+        // A child's bounds on one axis depend only on its index on that axis
+        // (create_children), so the children containing p are a product of
+        // per-axis index sets, and the first of them in creation order takes
+        // on each axis the first interval containing p: read it off the
+        // first child's bounds, then test the one child so found.
         const Node &nd = m.nodes[cur];
-        int64_t next = -1;
-        for (int c = 0; c < nd.nchild; ++c)
-            if (inside(m.nodes[nd.child0 + c].b, p)) {
-                next = nd.child0 + c;
-                break;
-            }
-        if (next < 0)
+        const Node &c0 = m.nodes[nd.child0];
+        const int cx = (nd.mask & 1) ? 2 : 1, cy = (nd.mask & 2) ? 2 : 1;
+        const int ix = (cx == 2 && p[0] > c0.b[1]) ? 1 : 0;
+        const int iy = (cy == 2 && p[1] > c0.b[3]) ? 1 : 0;
+        const int iz = ((nd.mask & 4) && p[2] > c0.b[5]) ? 1 : 0;
+        const int64_t next = nd.child0 + (iz * cy + iy) * cx + ix;
+        if (!inside(m.nodes[next].b, p))
             break;
         cur = next;
     }
@@ -168,10 +206,11 @@ int64_t neighbour_node(const drf_mesh &m, int64_t id, int f)
 }
 
 // Every leaf across face f of node id that shares a piece of the face of
-// positive area, in depth-first (element) order: a descent from the root
-// into the children that reach across the face plane and overlap the face
-// on both tangent axes (the 2:1 mortar connectivity, not the reference's).
-void adjacent_leaves(const drf_mesh &m, int64_t id, int f, std::vector<int64_t> &out)
+// positive area, in depth-first (element) order: a descent into the
+// children that reach across the face plane and overlap the face on both
+// tangent axes (the 2:1 mortar connectivity, not the reference's).
+void adjacent_leaves(const drf_mesh &m, int64_t id, int f, std::vector<int64_t> &out,
+                     std::vector<int64_t> &stack)
 {
     out.clear();
     const double *b = m.nodes[id].b;
@@ -187,7 +226,25 @@ void adjacent_leaves(const drf_mesh &m, int64_t id, int f, std::vector<int64_t> 
         const double lo = std::max(x[2 * t], b[2 * t]), hi = std::min(x[2 * t + 1], b[2 * t + 1]);
         return hi - lo > 1e-12 * (b[2 * t + 1] - b[2 * t]);
     };
-    std::vector<int64_t> stack{0};
+    // Every such leaf meets the interior of the same-size box across the
+    // face next to the face, so it lies in any node containing that box
+    // (the leaves partition each node): the descent starts at the nearest
+    // ancestor that does (the root at the latest) and lists what a descent
+    // from the root would, in the same order.  A bound that rounding moves
+    // off the box only sends the search one ancestor further up.
+    double box[6];
+    std::memcpy(box, b, sizeof(box));
+    box[2 * a] = (f % 2) ? plane : 2 * plane - b[2 * a + 1];
+    box[2 * a + 1] = (f % 2) ? 2 * plane - b[2 * a] : plane;
+    int64_t top = m.nodes[id].parent;
+    while (top > 0) {
+        const double *t = m.nodes[top].b;
+        if (t[0] <= box[0] && t[1] >= box[1] && t[2] <= box[2] && t[3] >= box[3] &&
+            t[4] <= box[4] && t[5] >= box[5])
+            break;
+        top = m.nodes[top].parent;
+    }
+    stack.assign(1, top > 0 ? top : 0);
     while (!stack.empty()) {
         const int64_t cur = stack.back();
         stack.pop_back();
@@ -233,12 +290,23 @@ template <class F> void parallel_chunks(int64_t n, F fn)
 // Does leaf id have a face neighbour more than one level coarser on some axis?
 bool needs_split(const drf_mesh &m, int64_t id)
 {
+    const Node &nd = m.nodes[id];
+    const Node *par = nd.parent >= 0 ? &m.nodes[nd.parent] : nullptr;
     for (int f = 0; f < 6; ++f) {
+        // a face between two children of the parent has a sibling of the
+        // same size across it (or that sibling's finer leaves), never a
+        // coarser leaf; the lower child's lower bound is the parent's
+        const int a = f / 2;
+        if (par && (par->mask >> a & 1)) {
+            const bool upper_child = nd.b[2 * a] != par->b[2 * a];
+            if ((f % 2 == 1) != upper_child)
+                continue;
+        }
         const int64_t nb = neighbour_node(m, id, f);
         if (nb < 0 || m.nodes[nb].nchild)
             continue;
-        for (int a = 0; a < 3; ++a)
-            if (m.nodes[id].lev[a] - m.nodes[nb].lev[a] > 1)
+        for (int ax = 0; ax < 3; ++ax)
+            if (nd.lev[ax] - m.nodes[nb].lev[ax] > 1)
                 return true;
     }
     return false;
@@ -305,7 +373,7 @@ void balance(drf_mesh &m, int64_t first_new = 0)
             // next pass: the leaves this pass created, and the leaves that
             // split a neighbour (one split narrows a level gap by one only,
             // so they may still need another), in leaf order
-            number_leaves(m);
+            renumber_leaves(m);
             std::vector<char> again(m.nodes.size(), 0);
             for (int64_t id : split_by)
                 again[id] = 1;
@@ -416,7 +484,7 @@ drf_status drf_mesh_refine(drf_mesh *m, int64_t count, const int64_t *elems, con
         m->nodes[id].mask = masks[i] & 7;
         create_children(*m, id);
     }
-    number_leaves(*m);
+    renumber_leaves(*m);
     balance(*m, first_new);
     return DRF_OK;
 }
@@ -470,6 +538,7 @@ drf_status face_connections(const drf_mesh *m, bool at_level, uint8_t *type, int
     const int64_t E = (int64_t)m->leaves.size();
     // elements are independent: each writes its own six faces
     parallel_chunks(E, [&](int64_t e0, int64_t e1) {
+    std::vector<int64_t> adj, stack, fine; // reused by the chunk's queries
     for (int64_t e = e0; e < e1; ++e) {
         const int64_t id = m->leaves[e];
         const Node &nd = m->nodes[id];
@@ -489,8 +558,7 @@ drf_status face_connections(const drf_mesh *m, bool at_level, uint8_t *type, int
                 continue;
             const int8_t opp = (int8_t)((f % 2 == 0) ? f + 1 : f - 1);
             if (at_level) {
-                std::vector<int64_t> adj;
-                adjacent_leaves(*m, id, f, adj);
+                adjacent_leaves(*m, id, f, adj, stack);
                 if (adj.empty())
                     continue;
                 const Node &x = m->nodes[adj[0]];
@@ -520,7 +588,9 @@ drf_status face_connections(const drf_mesh *m, bool at_level, uint8_t *type, int
                 continue;
             }
             // fine leaves sharing the face, breadth-first from the node (:582-639)
-            std::vector<int64_t> q{nb}, fine;
+            std::vector<int64_t> &q = stack;
+            q.assign(1, nb);
+            fine.clear();
             const double plane = (f % 2) ? nd.b[2 * a + 1] : nd.b[2 * a];
             for (size_t qh = 0; qh < q.size(); ++qh) {
                 const Node &x = m->nodes[q[qh]];
