@@ -31,6 +31,10 @@ struct Shape {
     static constexpr int N2 = N * N, ND = N2 * N;
     static constexpr int EPB = ND >= 1024 ? 1 : 1024 / ND;       // outputs per batch (boxes)
     static constexpr int PPB = 8 * ND >= 2048 ? 1 : 2048 / (8 * ND); // parents per batch
+    // outputs per batch of the box interpolation: as many elements as the
+    // workgroup has lanes for their grid lines (10 of 25 lines at N = 5,
+    // 250 lanes busy, where 1,024 nodes per batch kept 200)
+    static constexpr int BPB = N2 >= kThreads ? 1 : kThreads / N2;
 };
 
 __device__ __forceinline__ int64_t parent_source(BoxMode mode, int64_t p,
@@ -241,7 +245,7 @@ __device__ __forceinline__ int interval_kind(double lo, double hi)
 // the eta and zeta passes read the line they contract from LDS, and the
 // matrices are read from LDS (holding an element's three matrices in
 // registers for the batch measured slower at N = 5: 204 VGPRs and 2 waves
-// per SIMD, against 70 and 7).  Every output is the same sum, in the same
+// per SIMD, against 70 VGPRs and the 5 workgroups per CU that LDS allows).  Every output is the same sum, in the same
 // order, as a pass that makes one output per thread.
 template <int N>
 __global__ void __launch_bounds__(kThreads)
@@ -250,7 +254,7 @@ __global__ void __launch_bounds__(kThreads)
                         NodeSet ns)
 {
     using S = Shape<N>;
-    constexpr int N2 = S::N2, ND = S::ND, EPB = S::EPB;
+    constexpr int N2 = S::N2, ND = S::ND, EPB = S::BPB;
     constexpr int SL = (EPB * N2 + kThreads - 1) / kThreads; // grid lines per thread
     __shared__ double sT[EPB * ND], sV[EPB * ND], sM[EPB * 3 * N2], sStd[3 * N2], sBox[EPB * 6];
     __shared__ int64_t sSrc[EPB];
@@ -495,7 +499,7 @@ hipError_t interp_n(BoxMode mode, int mask, int64_t n_out, int64_t n_src, const 
                            blk, 0, s, mask, n_src, n_src, U, es, (int64_t)0, idx, ns, out, os);
         break;
     default:
-        hipLaunchKernelGGL((interp_boxes_kernel<N>), dim3(grid_for<S::EPB>(n_out)), blk, 0, s,
+        hipLaunchKernelGGL((interp_boxes_kernel<N>), dim3(grid_for<S::BPB>(n_out)), blk, 0, s,
                            n_out, n_src, bf, idx, boxes, ns);
         break;
     }
