@@ -113,3 +113,29 @@ def test_deep_local_refinement_cascades(orc, seed):
         o.refine(sel, masks)
     assert p.num_elements() > 4096
     _compare(p, o)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_refinement_on_a_domain_with_inexact_bounds(orc, seed):
+    """Bounds that are not dyadic fractions (a child's upper bound computed
+    from its lower bound and width can differ from its parent's by rounding,
+    and probe points can fall on such bounds): the neighbour descent that
+    picks one child per level from the first child's bounds, the balance
+    filter that skips faces between siblings and the 2:1 search that starts
+    below the root must give what the oracle's scans over every child give."""
+    rng = np.random.default_rng(seed)
+    dom = (0.1, 12345.678, -3.3, 777.7, -1.7, -0.013)
+    p = mesh.OctreeAdapter(*dom)
+    o = orc.Mesh(*dom)
+    p.build_uniform(12, 10, 3)
+    o.build_uniform(12, 10, 3)
+    for _ in range(4):
+        E = p.num_elements()
+        sel = np.unique(rng.integers(0, E, size=max(1, E // 5)))
+        masks = rng.integers(1, 8, size=sel.size)
+        p.refine(sel, masks)
+        o.refine(sel, masks)
+        _compare(p, o)
+    p.balance()
+    o.balance()
+    _compare(p, o)
