@@ -240,7 +240,7 @@ def test_interpolate_boxes_any_boxes_invalid_sources_grid_stride(torch, orc, ord
 
     n = order + 1
     nd = n ** 3
-    epb = max(1024 // nd, 1)
+    epb = max(256 // (n * n), 1)  # elements per batch: one per lane's grid line
     n_src, n_out = 300, 4096 * epb + 777
     rng = np.random.default_rng(order)
     U = rng.standard_normal((n_src, nd))
@@ -284,7 +284,7 @@ def test_interpolate_boxes_fields_bit_identical_to_one_field(torch, order):
 
     n = order + 1
     nd = n ** 3
-    epb = max(1024 // nd, 1)
+    epb = max(256 // (n * n), 1)  # elements per batch: one per lane's grid line
     n_src, n_out = 257, 4096 * epb + 131
     rng = np.random.default_rng(100 + order)
     P = torch.from_numpy(rng.standard_normal((n_src, 5 * nd))).cuda()
